@@ -1,0 +1,163 @@
+// Path-vertex building blocks shared by the megakernel, the wavefront
+// stages, the adjoint and the batched renderer.
+//
+// One path sample follows Renderer::ProcessSample_PT (src/nanogi.cpp:446-607)
+// and Renderer::ProcessSample_PTDirect (src/nanogi.cpp:609-802):
+//   camera vertex (numVertices = 1) -> [NEE] -> sample direction -> trace ->
+//   [emission on hit, pt only] -> Russian roulette -> next vertex,
+// with random numbers drawn from the counter-based stream of ngi_common.h.
+#pragma once
+
+#include <float.h>
+
+#include "ngi_shade.h"
+#include "ngi_traverse.h"
+
+namespace ngi {
+
+#define NGI_EPS_F 1e-4f   // EpsF (include/nanogi/basic.hpp:85)
+// Hard cap on path vertices so every lane terminates even with -m -1 and
+// Russian roulette off (the reference would loop forever in a closed scene).
+#define NGI_MAX_VERTICES 1024
+
+enum { CNT_EXT_RAYS = 0, CNT_SHADOW_RAYS, CNT_EXT_NODES, CNT_EXT_TRIS, CNT_SH_NODES, CNT_SH_TRIS, CNT_SAMPLES, CNT_N };
+
+struct RenderArgs {
+    DevScene sc;
+    void* film;                 // FilmT[W*H*3]
+    int w, h;
+    uint32_t npix;
+    int sampling;               // NGI_SAMPLING_*
+    uint64_t path_begin;        // first path id of the launch
+    uint64_t path_count;
+    uint32_t sample_base;       // per-pixel: sample index of path id 0
+    int max_nv;
+    int rr_depth;
+    float rr_prob_f;
+    double rr_prob_d;
+    uint32_t seed, stream_id;
+    unsigned long long* counters;   // CNT_N entries or null
+};
+
+template <typename R> __device__ __forceinline__ R unif(uint32_t x);
+template <> __device__ __forceinline__ float unif<float>(uint32_t x) { return u24f(x); }
+template <> __device__ __forceinline__ double unif<double>(uint32_t x) { return u24d(x); }
+template <typename R> __device__ __forceinline__ R rr_prob(const RenderArgs& a);
+template <> __device__ __forceinline__ float rr_prob<float>(const RenderArgs& a) { return a.rr_prob_f; }
+template <> __device__ __forceinline__ double rr_prob<double>(const RenderArgs& a) { return a.rr_prob_d; }
+
+__device__ __forceinline__ U4 rng4(uint32_t c0, uint32_t c1, uint32_t vertex, uint32_t stream, uint32_t seed, uint32_t sid) {
+    U4 c;
+    c.x = c0, c.y = c1, c.z = vertex, c.w = stream;
+    return philox4x32_10(c, seed, sid);
+}
+
+template <typename FilmT, typename R>
+__device__ __forceinline__ void film_add(void* film, int pixel, V3<R> v) {
+    FilmT* f = reinterpret_cast<FilmT*>(film) + 3 * (size_t)pixel;
+    if (v.x != R(0)) unsafeAtomicAdd(f + 0, (FilmT)v.x);
+    if (v.y != R(0)) unsafeAtomicAdd(f + 1, (FilmT)v.y);
+    if (v.z != R(0)) unsafeAtomicAdd(f + 2, (FilmT)v.z);
+}
+
+template <int WIDTH, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, TraceHit& h, TraceCounters& c, uint32_t* stk, int stride) {
+    if (WIDTH == 8) return trace_bvh8<ANY, COUNT>(sc, r, h, c, stk, stride);
+    return trace_bvh2<ANY, COUNT>(sc, r, h, c);
+}
+
+// Surface at a hit: p = o + d t in the shading precision (rt.hpp:2197),
+// geometric normal from the winding (2206), shading normal interpolated
+// from vertex normals with weights (1 - u - v, u, v) computed in FP32 as
+// Embree's u, v are (2209-2218), NaN -> geometric normal.
+template <typename R>
+__device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R> d, const TraceHit& h, int& prim) {
+    const float4* tv = sc.tris + 3 * (size_t)h.leaf;
+    const float4* sh = sc.shade + 3 * (size_t)h.leaf;
+    const float4 v0 = tv[0], v1 = tv[1], v2 = tv[2];
+    const float4 n0 = sh[0], n1 = sh[1], n2 = sh[2];
+    prim = __float_as_int(n0.w);
+    Surf<R> s;
+    s.p = o + d * (R)h.t;
+    const V3<R> p1((R)v0.x, (R)v0.y, (R)v0.z), p2((R)v1.x, (R)v1.y, (R)v1.z), p3((R)v2.x, (R)v2.y, (R)v2.z);
+    s.gn = vnorm(cross(p2 - p1, p3 - p1));
+    const R w0 = (R)(1.0f - h.u - h.v);
+    V3<R> sn = V3<R>((R)n0.x, (R)n0.y, (R)n0.z) * w0 + V3<R>((R)n1.x, (R)n1.y, (R)n1.z) * (R)h.u +
+               V3<R>((R)n2.x, (R)n2.y, (R)n2.z) * (R)h.v;
+    sn = vnorm(sn);
+    if (Math<R>::isnan_(sn.x) || Math<R>::isnan_(sn.y) || Math<R>::isnan_(sn.z)) sn = s.gn;
+    s.f = make_frame(sn);
+    s.degenerate = false;
+    return s;
+}
+
+// Next-event estimation at one vertex (src/nanogi.cpp:656-710).  Returns the
+// unoccluded contribution C (zero if nothing to connect) and fills the
+// shadow ray and the film pixel it splats to.
+template <typename R>
+__device__ __forceinline__ V3<R> nee_contribution(const DevScene& sc, const Cam<R>& cam, const DevPrim<R>* prims, int type, int prim,
+                                                  const Surf<R>& s, V3<R> wi, V3<R> T, U4 rn, int w, int h, int pixel, TraceRay& sr,
+                                                  int& target) {
+    target = pixel;
+    if (sc.num_lights == 0) return V3<R>(R(0));
+    const LightSample<R> ls = sample_light<R>(sc, unif<R>(rn.x), unif<R>(rn.y), unif<R>(rn.z));
+    if (!ls.valid) return V3<R>(R(0));
+    const V3<R> dv = ls.p - s.p;
+    const R d2 = dot(dv, dv);
+    const R len = Math<R>::sqrt_(d2);
+    const V3<R> ppL = dv / len;
+    V3<R> fsE;
+    if (type == NGI_PRIM_E) {
+        R rx = 0, ry = 0;
+        const R imp = camera_importance(cam, ppL, rx, ry);
+        if (imp == R(0)) return V3<R>(R(0));
+        fsE = V3<R>(imp);
+        target = pixel_index(rx, ry, w, h);
+    } else {
+        fsE = bsdf_eval_nee(prims[prim], type, s, wi, ppL);
+    }
+    if (!ls.degenerate && dot(ls.n, -ppL) <= R(0)) return V3<R>(R(0));   // area emitter back side
+    R G = R(1);
+    if (!s.degenerate) G *= Math<R>::abs_(dot(s.f.n, ppL));
+    if (!ls.degenerate) G *= Math<R>::abs_(dot(ls.n, -ppL));
+    G /= d2;
+    const V3<R> C = T * fsE * ls.Le * (G * ls.inv_pdf);
+    sr.ox = (float)s.p.x, sr.oy = (float)s.p.y, sr.oz = (float)s.p.z;
+    sr.dx = (float)ppL.x, sr.dy = (float)ppL.y, sr.dz = (float)ppL.z;
+    sr.tmin = NGI_EPS_F;
+    sr.tmax = (float)len * (1.0f - NGI_EPS_F);   // Scene::Visible (rt.hpp:2251-2261)
+    return C;
+}
+
+// Per-path sample coordinates: RNG counter words and the pixel (per-pixel
+// sampling) or -1 (pixel decided by the camera sample).
+__device__ __forceinline__ void path_coords(const RenderArgs& a, uint64_t id, uint32_t& c0, uint32_t& c1, int& pixel) {
+    if (a.sampling == NGI_SAMPLING_PER_PIXEL) {
+        pixel = (int)(id % a.npix);
+        c0 = (uint32_t)pixel;
+        c1 = a.sample_base + (uint32_t)(id / a.npix);
+    } else {
+        pixel = -1;
+        c0 = (uint32_t)id;
+        c1 = (uint32_t)(id >> 32) | NGI_UNIFORM_FLAG;
+    }
+}
+
+// Camera vertex direction sample (rt.hpp:733-740); fs/pdf of the pinhole is
+// exactly 1 (EvaluateDirection :955-978 equals EvaluateDirectionPDF
+// :1189-1212), so the throughput stays 1.
+template <typename R>
+__device__ __forceinline__ V3<R> camera_sample(const RenderArgs& a, const Cam<R>& cam, R u0, R u1, int& pixel) {
+    R rx, ry;
+    if (a.sampling == NGI_SAMPLING_PER_PIXEL) {
+        const int px = pixel % a.w, py = pixel / a.w;
+        rx = ((R)px + u0) / (R)a.w;
+        ry = ((R)py + u1) / (R)a.h;
+    } else {
+        rx = u0, ry = u1;
+        pixel = pixel_index(rx, ry, a.w, a.h);
+    }
+    return camera_dir(cam, rx, ry);
+}
+
+}  // namespace ngi

@@ -1,0 +1,469 @@
+// Rendering entry points: the one-path-per-lane megakernel and the render
+// session (film, statistics, batch moments).  The wavefront engine lives in
+// ngi_wavefront.hip and shares the session.
+//
+// Replaces Renderer::Render / RenderProcess (src/nanogi.cpp:182-440): the TBB
+// loop over samples becomes launches over path ids; per-thread FP64 films
+// become one FP64 film updated with hardware FP64 atomics; the final
+// W*H/N scaling (src/nanogi.cpp:434-437) is applied on read-back.
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ngi_path.h"
+#include "ngi_render.h"
+
+namespace ngi {
+
+#define NGI_MK_BLOCK 256
+
+template <typename T>
+__device__ __forceinline__ void wave_add(unsigned long long* p, T v) {
+    unsigned long long x = (unsigned long long)v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(p, x);
+}
+
+// One lane = one path sample (ProcessSample_PT / ProcessSample_PTDirect).
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+__global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
+    __shared__ uint32_t stk[WIDTH == 8 ? NGI_BVH8_STACK * NGI_MK_BLOCK : 1];
+    const uint64_t local = (uint64_t)blockIdx.x * NGI_MK_BLOCK + threadIdx.x;
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    if (local < a.path_count) {
+        const DevScene& sc = a.sc;
+        const uint64_t id = a.path_begin + local;
+        uint32_t c0, c1;
+        int pixel;
+        path_coords(a, id, c0, c1, pixel);
+        const Cam<R> cam = load_cam<R>(sc.cam);
+        const DevPrim<R>* prims = prims_of<R>(sc);
+        V3<R> T(R(1));
+        int type = NGI_PRIM_E, prim = sc.sensor_prim;
+        Surf<R> s;
+        s.p = cam.eye;
+        s.degenerate = true;
+        s.gn = V3<R>(R(0));
+        s.f.n = s.f.s = s.f.t = V3<R>(R(0));
+        V3<R> wi(R(0));
+        int nv = 1;
+        const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
+        cnt[CNT_SAMPLES] = 1;
+        uint32_t* my = stk + threadIdx.x;
+        for (;;) {
+            if (nv >= max_nv) break;
+            if (DIRECT) {
+                const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
+                TraceRay sr;
+                int target;
+                const V3<R> C = nee_contribution<R>(sc, cam, prims, type, prim, s, wi, T, rn, a.w, a.h, pixel, sr, target);
+                if (!is_zero(C)) {
+                    TraceHit sh;
+                    TraceCounters tc{0, 0};
+                    const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, my, NGI_MK_BLOCK);
+                    cnt[CNT_SHADOW_RAYS]++;
+                    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+                    if (!occluded) film_add<FilmT>(a.film, target, C);
+                }
+            }
+            const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
+            const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
+            V3<R> wo;
+            if (type == NGI_PRIM_E) {
+                wo = camera_sample<R>(a, cam, u0, u1, pixel);
+            } else {
+                V3<R> wgt;
+                if (!bsdf_sample<R>(prims[prim], type, s, wi, u0, u1, uc, wo, wgt)) break;
+                T = T * wgt;
+            }
+            TraceRay er;
+            er.ox = (float)s.p.x, er.oy = (float)s.p.y, er.oz = (float)s.p.z;
+            er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
+            er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
+            TraceHit h;
+            TraceCounters tc{0, 0};
+            const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, my, NGI_MK_BLOCK);
+            cnt[CNT_EXT_RAYS]++;
+            if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+            if (!hit) break;
+            int hp;
+            const Surf<R> ns = hit_surface<R>(sc, s.p, wo, h, hp);
+            const DevPrim<R>& hm = prims[hp];
+            if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
+                // Emission on hit (src/nanogi.cpp:568-575): area lights only.
+                const DevLight<R>& L = lights_of<R>(sc)[hm.light];
+                if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -wo) > R(0))
+                    film_add<FilmT>(a.film, pixel, T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
+            }
+            if (a.rr_depth >= 0 && nv >= a.rr_depth) {   // src/nanogi.cpp:583-591
+                const R q = rr_prob<R>(a);
+                if (urr > q) break;
+                T = T / q;
+            }
+            type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+            if (type == 0) break;   // [L]-only primitives absorb (rt.hpp:1146-1147)
+            s = ns;
+            prim = hp;
+            wi = -wo;
+            nv++;
+        }
+    }
+    if (a.counters) {
+        for (int k = 0; k < CNT_N; ++k)
+            if (COUNT || (k != CNT_EXT_NODES && k != CNT_EXT_TRIS && k != CNT_SH_NODES && k != CNT_SH_TRIS)) wave_add(a.counters + k, cnt[k]);
+    }
+}
+
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+static void launch_mk(const RenderArgs& a, hipStream_t st) {
+    const unsigned grid = (unsigned)((a.path_count + NGI_MK_BLOCK - 1) / NGI_MK_BLOCK);
+    megakernel<R, FilmT, DIRECT, WIDTH, COUNT><<<grid, NGI_MK_BLOCK, 0, st>>>(a);
+}
+
+template <typename R, typename FilmT, bool COUNT>
+static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t st) {
+    if (direct) {
+        if (width == 8) launch_mk<R, FilmT, true, 8, COUNT>(a, st);
+        else launch_mk<R, FilmT, true, 2, COUNT>(a, st);
+    } else {
+        if (width == 8) launch_mk<R, FilmT, false, 8, COUNT>(a, st);
+        else launch_mk<R, FilmT, false, 2, COUNT>(a, st);
+    }
+}
+
+void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st) {
+    if (precision == NGI_PRECISION_FP64_ALL) launch_mk3<double, double, false>(a, direct, width, st);
+    else if (precision == NGI_PRECISION_FP32_ALL) launch_mk3<float, float, false>(a, direct, width, st);
+    else if (count) launch_mk3<float, double, true>(a, direct, width, st);
+    else launch_mk3<float, double, false>(a, direct, width, st);
+}
+
+// ---------------------------------------------------------------------------
+// Film moments: m1 += batch, m2 += batch^2 (batch films normalised).
+// ---------------------------------------------------------------------------
+template <typename FilmT>
+__global__ void fold_batch(const FilmT* __restrict__ batch, double scale, double* __restrict__ m1, double* __restrict__ m2,
+                           FilmT* __restrict__ film, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double b = (double)batch[i];
+    const double v = b * scale;
+    m1[i] += v;
+    m2[i] += v * v;
+    film[i] += (FilmT)b;
+}
+
+template <typename FilmT>
+__global__ void zero_kernel(FilmT* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = FilmT(0);
+}
+
+int Session::zero(void* p, size_t n_elems, bool f64) {
+    if (n_elems == 0) return NGI_OK;
+    const unsigned grid = (unsigned)((n_elems + 255) / 256);
+    if (f64) zero_kernel<double><<<grid, 256, 0, stream()>>>(reinterpret_cast<double*>(p), n_elems);
+    else zero_kernel<float><<<grid, 256, 0, stream()>>>(reinterpret_cast<float*>(p), n_elems);
+    NGI_HIP_CHECK(hipGetLastError());
+    return NGI_OK;
+}
+
+int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
+    scene = sc;
+    p = pp;
+    if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT)
+        return fail(NGI_ERR_UNSUPPORTED, "renderer %d is not supported (only pt and ptdirect)", p.renderer);
+    if (p.width <= 0 || p.height <= 0 || (int64_t)p.width * p.height > (1ll << 28)) return fail(NGI_ERR_INVALID_ARG, "bad image size");
+    if (p.sampling != NGI_SAMPLING_PER_PIXEL && p.sampling != NGI_SAMPLING_REFERENCE_UNIFORM) return fail(NGI_ERR_INVALID_ARG, "bad sampling mode");
+    if (p.sampling == NGI_SAMPLING_REFERENCE_UNIFORM && p.num_samples <= 0) return fail(NGI_ERR_INVALID_ARG, "num_samples must be positive");
+    if (p.precision < 0 || p.precision > 2) return fail(NGI_ERR_INVALID_ARG, "bad precision");
+    if (p.engine != NGI_ENGINE_MEGAKERNEL && p.engine != NGI_ENGINE_WAVEFRONT) return fail(NGI_ERR_INVALID_ARG, "bad engine");
+    if (p.rr_depth >= 0 && !(p.rr_prob > 0.0 && p.rr_prob <= 1.0)) return fail(NGI_ERR_INVALID_ARG, "rr_prob must be in (0, 1]");
+    if (p.max_num_vertices < -1) return fail(NGI_ERR_INVALID_ARG, "bad max_num_vertices");
+    if (sc->sensor_prim < 0) return fail(NGI_ERR_INVALID_SCENE, "scene has no sensor (E) primitive");
+    if (sc->prims[sc->sensor_prim].sensor_type != NGI_SENSOR_PINHOLE) return fail(NGI_ERR_UNSUPPORTED, "only pinhole sensors are supported");
+    int st = device_prepare(*sc);
+    if (st) return st;
+    film64 = p.precision != NGI_PRECISION_FP32_ALL;
+    npix = (uint32_t)p.width * (uint32_t)p.height;
+    const size_t fb = (size_t)npix * 3 * (film64 ? 8 : 4);
+    NGI_HIP_CHECK(hipMalloc(&film, fb));
+    NGI_HIP_CHECK(hipMalloc(&counters, CNT_N * sizeof(unsigned long long)));
+    return reset();
+}
+
+int Session::reset() {
+    paths_done = 0;
+    samples_done = 0;
+    batches = 0;
+    memset(&stats, 0, sizeof(stats));
+    int st = zero(film, (size_t)npix * 3, film64);
+    if (st) return st;
+    if (m1) {
+        if ((st = zero(m1, (size_t)npix * 3, true))) return st;
+        if ((st = zero(m2, (size_t)npix * 3, true))) return st;
+    }
+    if ((st = zero(counters, CNT_N, true))) return st;   // 64-bit words
+    return sync();
+}
+
+void Session::destroy() {
+    if (scene && scene->dev) (void)hipStreamSynchronize(stream());
+    for (void* q : {film, counters, m1, m2, batch}) if (q) (void)hipFree(q);
+    film = counters = m1 = m2 = batch = nullptr;
+    release_engine();
+    for (auto& e : pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    pending.clear();
+}
+
+int Session::timed_begin(int kind, int64_t items, TimedLaunch& t) {
+    t.kind = kind;
+    t.items = items;
+    t.a = t.b = nullptr;
+    if (!p.time_kernels) return NGI_OK;
+    NGI_HIP_CHECK(hipEventCreate(&t.a));
+    NGI_HIP_CHECK(hipEventCreate(&t.b));
+    NGI_HIP_CHECK(hipEventRecord(t.a, stream()));
+    return NGI_OK;
+}
+
+int Session::timed_end(TimedLaunch& t) {
+    if (!t.a) return NGI_OK;
+    NGI_HIP_CHECK(hipEventRecord(t.b, stream()));
+    pending.push_back(t);
+    return NGI_OK;
+}
+
+int Session::sync() {
+    NGI_HIP_CHECK(hipStreamSynchronize(stream()));
+    for (auto& t : pending) {
+        float ms = 0;
+        NGI_HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+        switch (t.kind) {
+            case K_EXTEND: stats.ms_extend += ms, stats.extend_launches++, stats.extend_rays_timed += t.items; break;
+            case K_SHADOW: stats.ms_shadow += ms, stats.shadow_launches++; break;
+            case K_SHADE: stats.ms_shade += ms; break;
+            case K_MEGA: stats.ms_shade += ms; break;
+            default: break;
+        }
+        if (t.kind == K_EXTEND || t.kind == K_SHADOW) stats.ms_trace += ms, stats.trace_launches++;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+    return NGI_OK;
+}
+
+RenderArgs Session::args(void* target_film) const {
+    RenderArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sc = scene->dev->view;
+    a.film = target_film;
+    a.w = p.width, a.h = p.height;
+    a.npix = npix;
+    a.sampling = p.sampling;
+    a.max_nv = p.max_num_vertices;
+    a.rr_depth = p.rr_depth;
+    a.rr_prob_f = (float)p.rr_prob;
+    a.rr_prob_d = p.rr_prob;
+    a.seed = p.seed;
+    a.stream_id = p.stream_id;
+    a.counters = reinterpret_cast<unsigned long long*>(counters);
+    return a;
+}
+
+// Enqueue `spp` samples per pixel (or passes) into `target`.
+int Session::render_into(void* target, int spp) {
+    if (spp <= 0) return fail(NGI_ERR_INVALID_ARG, "spp must be positive");
+    int st = device_prepare(*scene);   // refresh parameters if they changed
+    if (st) return st;
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t total;
+    if (p.sampling == NGI_SAMPLING_PER_PIXEL) total = (uint64_t)npix * (uint64_t)spp;
+    else total = (uint64_t)p.num_samples * (uint64_t)spp;
+    if (p.engine == NGI_ENGINE_WAVEFRONT) {
+        st = render_wavefront(target, total);
+    } else {
+        // Launches of at most 2^22 paths (whole pixels rows of samples).
+        const uint64_t chunk = p.sampling == NGI_SAMPLING_PER_PIXEL ? (uint64_t)npix * std::max<uint64_t>(1, (1u << 22) / npix) : (1u << 22);
+        for (uint64_t done = 0; done < total && !st;) {
+            const uint64_t n = std::min<uint64_t>(chunk, total - done);
+            RenderArgs a = args(target);
+            if (p.sampling == NGI_SAMPLING_PER_PIXEL) {
+                a.path_begin = done;   // id -> (pixel = id % npix, sample = base + id / npix)
+                a.sample_base = samples_done;
+            } else {
+                a.path_begin = paths_done + done;
+            }
+            a.path_count = n;
+            TimedLaunch tl;
+            if ((st = timed_begin(K_MEGA, (int64_t)n, tl))) break;
+            launch_megakernel(a, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, scene->dev->view.bvh_width, p.collect_counters != 0, stream());
+            NGI_HIP_CHECK(hipGetLastError());
+            if ((st = timed_end(tl))) break;
+            done += n;
+        }
+    }
+    if (st) return st;
+    paths_done += total;
+    samples_done += (uint32_t)spp;
+    stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return NGI_OK;
+}
+
+int Session::render(int spp) { return render_into(film, spp); }
+
+int Session::render_batch(int spp) {
+    const size_t n = (size_t)npix * 3;
+    if (!m1) {
+        NGI_HIP_CHECK(hipMalloc(&m1, n * 8));
+        NGI_HIP_CHECK(hipMalloc(&m2, n * 8));
+        NGI_HIP_CHECK(hipMalloc(&batch, n * (film64 ? 8 : 4)));
+        int st = zero(m1, n, true);
+        if (!st) st = zero(m2, n, true);
+        if (st) return st;
+    }
+    int st = zero(batch, n, film64);
+    if (st) return st;
+    const uint64_t before = paths_done;
+    if ((st = render_into(batch, spp))) return st;
+    const double scale = (double)npix / (double)(paths_done - before);
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (film64) fold_batch<double><<<grid, 256, 0, stream()>>>(reinterpret_cast<double*>(batch), scale, reinterpret_cast<double*>(m1), reinterpret_cast<double*>(m2), reinterpret_cast<double*>(film), n);
+    else fold_batch<float><<<grid, 256, 0, stream()>>>(reinterpret_cast<float*>(batch), scale, reinterpret_cast<double*>(m1), reinterpret_cast<double*>(m2), reinterpret_cast<float*>(film), n);
+    NGI_HIP_CHECK(hipGetLastError());
+    batches++;
+    return NGI_OK;
+}
+
+int Session::get_film(double* out, double* var) {
+    int st = sync();
+    if (st) return st;
+    const size_t n = (size_t)npix * 3;
+    const double scale = paths_done ? (double)npix / (double)paths_done : 0.0;
+    if (out) {
+        if (film64) {
+            NGI_HIP_CHECK(hipMemcpy(out, film, n * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) out[i] *= scale;
+        } else {
+            std::vector<float> tmp(n);
+            NGI_HIP_CHECK(hipMemcpy(tmp.data(), film, n * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i] * scale;
+        }
+    }
+    if (var) {
+        if (!m1 || batches < 2) return fail(NGI_ERR_INVALID_ARG, "variance needs at least two batches (ngi_session_render_batch)");
+        std::vector<double> a(n), b(n);
+        NGI_HIP_CHECK(hipMemcpy(a.data(), m1, n * 8, hipMemcpyDeviceToHost));
+        NGI_HIP_CHECK(hipMemcpy(b.data(), m2, n * 8, hipMemcpyDeviceToHost));
+        const double k = (double)batches;
+        for (size_t i = 0; i < n; ++i) {
+            const double mean = a[i] / k;
+            const double sv = (b[i] - k * mean * mean) / (k - 1.0);   // sample variance of batch means
+            var[i] = (sv > 0 ? sv : 0.0) / k;
+        }
+    }
+    return NGI_OK;
+}
+
+int Session::get_stats(ngi_stats* out) {
+    int st = sync();
+    if (st) return st;
+    unsigned long long c[CNT_N];
+    NGI_HIP_CHECK(hipMemcpy(c, counters, sizeof(c), hipMemcpyDeviceToHost));
+    *out = stats;
+    out->samples = (int64_t)c[CNT_SAMPLES];
+    out->extension_rays = (int64_t)c[CNT_EXT_RAYS];
+    out->shadow_rays = (int64_t)c[CNT_SHADOW_RAYS];
+    out->node_visits = (double)c[CNT_EXT_NODES];
+    out->tri_tests = (double)c[CNT_EXT_TRIS];
+    out->shadow_node_visits = (double)c[CNT_SH_NODES];
+    out->shadow_tri_tests = (double)c[CNT_SH_TRIS];
+    return NGI_OK;
+}
+
+}  // namespace ngi
+
+using namespace ngi;
+
+struct ngi_session : ngi::Session {};
+
+extern "C" {
+
+int ngi_session_create(ngi_scene* scene, const ngi_render_params* p, ngi_session** out) {
+    clear_error();
+    if (!scene || !p || !out) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    ngi_session* s = new ngi_session();
+    int st = s->init(scene, *p);
+    if (st) {
+        s->destroy();
+        delete s;
+        return st;
+    }
+    *out = s;
+    return NGI_OK;
+}
+
+int ngi_session_render(ngi_session* s, int spp) {
+    clear_error();
+    if (!s) return fail(NGI_ERR_INVALID_ARG, "null session");
+    return s->render(spp);
+}
+
+int ngi_session_render_batch(ngi_session* s, int spp) {
+    clear_error();
+    if (!s) return fail(NGI_ERR_INVALID_ARG, "null session");
+    return s->render_batch(spp);
+}
+
+int ngi_session_sync(ngi_session* s) {
+    clear_error();
+    if (!s) return fail(NGI_ERR_INVALID_ARG, "null session");
+    return s->sync();
+}
+
+int ngi_session_get_film(ngi_session* s, double* film, double* var) {
+    clear_error();
+    if (!s) return fail(NGI_ERR_INVALID_ARG, "null session");
+    return s->get_film(film, var);
+}
+
+int ngi_session_get_stats(ngi_session* s, ngi_stats* out) {
+    clear_error();
+    if (!s || !out) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    return s->get_stats(out);
+}
+
+int ngi_session_reset(ngi_session* s) {
+    clear_error();
+    if (!s) return fail(NGI_ERR_INVALID_ARG, "null session");
+    return s->reset();
+}
+
+void ngi_session_destroy(ngi_session* s) {
+    if (!s) return;
+    s->destroy();
+    delete s;
+}
+
+int ngi_render(ngi_scene* scene, const ngi_render_params* p, double* film, ngi_stats* stats) {
+    clear_error();
+    if (!scene || !p || !film) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    ngi_session* s = nullptr;
+    int st = ngi_session_create(scene, p, &s);
+    if (st) return st;
+    const int passes = p->sampling == NGI_SAMPLING_PER_PIXEL ? p->spp : 1;
+    st = s->render(passes);
+    if (!st) st = s->get_film(film, nullptr);
+    if (!st && stats) st = s->get_stats(stats);
+    const std::string err = st ? std::string(ngi_last_error()) : std::string();
+    ngi_session_destroy(s);
+    if (st) fail(st, "%s", err.c_str());
+    return st;
+}
+
+}  // extern "C"

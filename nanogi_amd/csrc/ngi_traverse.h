@@ -1,0 +1,280 @@
+// FP32 closest-hit and any-hit BVH traversal (device).
+//
+// Replaces rtcIntersect inside Scene::Intersect / Scene::Visible
+// (include/nanogi/rt.hpp:2162-2261): two-sided triangles, hit interval
+// (tmin, tmax) exclusive, barycentrics (u, v) = weights of vertices 1 and 2.
+//
+// Triangle test: watertight ray/triangle intersection (Woop, Benthin, Wald,
+// "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013): shear to ray
+// space, scaled edge functions with a double-precision recomputation when
+// an edge function is exactly zero.  Box test: slabs with the far distance
+// scaled up by 1 + 2*gamma(3) (Ize, "Robust BVH Ray Traversal", JCGT 2013)
+// so traversal is conservative with respect to the triangle test.
+#pragma once
+
+#include "ngi_dev.h"
+
+namespace ngi {
+
+struct TraceRay {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float tmin, tmax;
+};
+
+struct TraceHit {
+    float t, u, v;
+    int32_t leaf;   // leaf-order triangle index, -1 = miss
+};
+
+struct TraceCounters {
+    uint32_t nodes;
+    uint32_t tris;
+};
+
+// Per-ray constants for the watertight test and the slab test.
+struct RayPre {
+    float ox, oy, oz;
+    float ix, iy, iz;        // 1 / d (guarded against zero components)
+    float sx, sy, sz;        // shear constants
+    int kx, ky, kz;
+    float dx, dy, dz;
+    uint32_t octant;         // bit k set if d[k] < 0
+};
+
+__device__ __forceinline__ float guard_dir(float d) {
+    return fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+}
+
+__device__ __forceinline__ RayPre make_pre(const TraceRay& r) {
+    RayPre p;
+    p.ox = r.ox, p.oy = r.oy, p.oz = r.oz;
+    p.dx = r.dx, p.dy = r.dy, p.dz = r.dz;
+    p.ix = 1.0f / guard_dir(r.dx);
+    p.iy = 1.0f / guard_dir(r.dy);
+    p.iz = 1.0f / guard_dir(r.dz);
+    const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float d[3] = {r.dx, r.dy, r.dz};
+    if (d[kz] < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    p.kx = kx, p.ky = ky, p.kz = kz;
+    p.sx = d[kx] / d[kz];
+    p.sy = d[ky] / d[kz];
+    p.sz = 1.0f / d[kz];
+    p.octant = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    return p;
+}
+
+__device__ __forceinline__ float pick(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+// Watertight test; on a hit inside (tmin, tmax) writes t, u, v.
+__device__ __forceinline__ bool tri_test(const RayPre& r, float4 a4, float4 b4, float4 c4, float tmin, float tmax, float& t,
+                                         float& u, float& v) {
+#pragma clang fp contract(off)
+    const float Ax0 = a4.x - r.ox, Ay0 = a4.y - r.oy, Az0 = a4.z - r.oz;
+    const float Bx0 = b4.x - r.ox, By0 = b4.y - r.oy, Bz0 = b4.z - r.oz;
+    const float Cx0 = c4.x - r.ox, Cy0 = c4.y - r.oy, Cz0 = c4.z - r.oz;
+    const float Akx = pick(Ax0, Ay0, Az0, r.kx), Aky = pick(Ax0, Ay0, Az0, r.ky), Akz = pick(Ax0, Ay0, Az0, r.kz);
+    const float Bkx = pick(Bx0, By0, Bz0, r.kx), Bky = pick(Bx0, By0, Bz0, r.ky), Bkz = pick(Bx0, By0, Bz0, r.kz);
+    const float Ckx = pick(Cx0, Cy0, Cz0, r.kx), Cky = pick(Cx0, Cy0, Cz0, r.ky), Ckz = pick(Cx0, Cy0, Cz0, r.kz);
+    const float Ax = Akx - r.sx * Akz, Ay = Aky - r.sy * Akz;
+    const float Bx = Bkx - r.sx * Bkz, By = Bky - r.sy * Bkz;
+    const float Cx = Ckx - r.sx * Ckz, Cy = Cky - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    // Sign-corrected range test: tmin < T/det < tmax.
+    const float sT = det < 0.0f ? -T : T;
+    const float sd = fabsf(det);
+    if (sT <= tmin * sd || sT >= tmax * sd) return false;
+    const float rd = 1.0f / det;
+    t = T * rd;
+    if (!(t > tmin && t < tmax)) return false;
+    u = V * rd;
+    v = W * rd;
+    return true;
+}
+
+#define NGI_SLAB_SCALE 1.00000036f   // 1 + 2*gamma(3), gamma(n) = n*eps/(1-n*eps), eps = 2^-24
+
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ void leaf_test(const DevScene& sc, const RayPre& r, int32_t ref, float tmin, float& tmax, TraceHit& h,
+                                          TraceCounters& cnt, bool& done) {
+    const uint32_t v = (uint32_t)(~ref);
+    const uint32_t first = v >> 4, count = (v & 15u) + 1u;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t leaf = first + k;
+        const float4 a = sc.tris[3 * leaf], b = sc.tris[3 * leaf + 1], c = sc.tris[3 * leaf + 2];
+        if (COUNT) cnt.tris++;
+        float t, u, w;
+        if (tri_test(r, a, b, c, tmin, tmax, t, u, w)) {
+            tmax = t;
+            h.t = t, h.u = u, h.v = w, h.leaf = (int32_t)leaf;
+            if (ANY) {
+                done = true;
+                return;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Binary BVH, private stack (depth is bounded by the builder).
+// ---------------------------------------------------------------------------
+#define NGI_BVH2_STACK 64
+
+template <bool ANY, bool COUNT>
+__device__ bool trace_bvh2(const DevScene& sc, const TraceRay& ray, TraceHit& h, TraceCounters& cnt) {
+    const RayPre r = make_pre(ray);
+    const float tmin = ray.tmin;
+    float tmax = ray.tmax;
+    h.leaf = -1;
+    int32_t stack[NGI_BVH2_STACK];
+    int sp = 0;
+    int32_t node = 0;
+    bool done = false;
+    for (;;) {
+        if (node >= 0) {
+            const float4* n = sc.nodes + 4 * (size_t)node;
+            const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+            if (COUNT) cnt.nodes++;
+            float x0 = (n0.x - r.ox) * r.ix, x1 = (n0.y - r.ox) * r.ix;
+            float y0 = (n0.z - r.oy) * r.iy, y1 = (n0.w - r.oy) * r.iy;
+            float z0 = (n2.x - r.oz) * r.iz, z1 = (n2.y - r.oz) * r.iz;
+            const float l0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            const float f0 = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * NGI_SLAB_SCALE, tmax);
+            x0 = (n1.x - r.ox) * r.ix, x1 = (n1.y - r.ox) * r.ix;
+            y0 = (n1.z - r.oy) * r.iy, y1 = (n1.w - r.oy) * r.iy;
+            z0 = (n2.z - r.oz) * r.iz, z1 = (n2.w - r.oz) * r.iz;
+            const float l1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            const float f1 = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * NGI_SLAB_SCALE, tmax);
+            const bool hit0 = l0 <= f0;
+            const bool hit1 = l1 <= f1;
+            const int32_t ch0 = __float_as_int(n3.x), ch1 = __float_as_int(n3.y);
+            if (hit0 && hit1) {
+                int32_t near = ch0, far = ch1;
+                if (l1 < l0) {
+                    near = ch1;
+                    far = ch0;
+                }
+                stack[sp++] = far;
+                node = near;
+            } else if (hit0) {
+                node = ch0;
+            } else if (hit1) {
+                node = ch1;
+            } else {
+                if (sp == 0) break;
+                node = stack[--sp];
+            }
+        } else {
+            leaf_test<ANY, COUNT>(sc, r, node, tmin, tmax, h, cnt, done);
+            if (ANY && done) return true;
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    return h.leaf >= 0;
+}
+
+// ---------------------------------------------------------------------------
+// 8-wide BVH.  Child slots are assigned at build time by the octant of the
+// child's centroid relative to the parent's; a ray visits hit children in
+// slot order XOR its direction octant (near-to-far along each axis).
+// The stack holds one entry per tree level: (node << 8) | pending mask.
+// Stack entries live in LDS, one column per lane (conflict-free).
+// ---------------------------------------------------------------------------
+#define NGI_BVH8_STACK 32
+
+__device__ __forceinline__ float exp2_bits(uint32_t biased) { return __uint_as_float(biased << 23); }
+
+template <bool COUNT>
+__device__ __forceinline__ uint32_t visit8(const DevScene& sc, const RayPre& r, uint32_t node, float tmin, float tmax,
+                                           TraceCounters& cnt) {
+    const float4* n = sc.nodes + 8 * (size_t)node;
+    const float4 h0 = n[0];
+    const uint4 q0 = __builtin_bit_cast(uint4, n[1]);  // qlox[8], qloy[8]
+    const uint4 q1 = __builtin_bit_cast(uint4, n[2]);  // qloz[8], qhix[8]
+    const uint4 q2 = __builtin_bit_cast(uint4, n[3]);  // qhiy[8], qhiz[8]
+    if (COUNT) cnt.nodes++;
+    const uint32_t meta = __float_as_uint(h0.w);
+    const float scx = exp2_bits(meta & 0xffu), scy = exp2_bits((meta >> 8) & 0xffu), scz = exp2_bits((meta >> 16) & 0xffu);
+    const uint32_t valid = meta >> 24;
+    // t(q) = (origin + q * scale - o) * invd = q * (scale * invd) + (origin - o) * invd
+    const float ax = scx * r.ix, bx = (h0.x - r.ox) * r.ix;
+    const float ay = scy * r.iy, by = (h0.y - r.oy) * r.iy;
+    const float az = scz * r.iz, bz = (h0.z - r.oz) * r.iz;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t wlox = k < 4 ? q0.x : q0.y, wloy = k < 4 ? q0.z : q0.w;
+        const uint32_t wloz = k < 4 ? q1.x : q1.y, whix = k < 4 ? q1.z : q1.w;
+        const uint32_t whiy = k < 4 ? q2.x : q2.y, whiz = k < 4 ? q2.z : q2.w;
+        const int sh = (k & 3) * 8;
+        const float lx = (float)((wlox >> sh) & 0xffu), hx = (float)((whix >> sh) & 0xffu);
+        const float ly = (float)((wloy >> sh) & 0xffu), hy = (float)((whiy >> sh) & 0xffu);
+        const float lz = (float)((wloz >> sh) & 0xffu), hz = (float)((whiz >> sh) & 0xffu);
+        const float tx0 = lx * ax + bx, tx1 = hx * ax + bx;
+        const float ty0 = ly * ay + by, ty1 = hy * ay + by;
+        const float tz0 = lz * az + bz, tz1 = hz * az + bz;
+        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        const float tfa = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * NGI_SLAB_SCALE;
+        const bool hit = tn <= fminf(tfa, tmax) && ((valid >> k) & 1u);
+        // processing index i = slot ^ octant
+        mask |= (hit ? 1u : 0u) << (k ^ r.octant);
+    }
+    return mask;
+}
+
+template <bool ANY, bool COUNT>
+__device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h, TraceCounters& cnt, uint32_t* lds_stack,
+                           int lane_stride) {
+    const RayPre r = make_pre(ray);
+    const float tmin = ray.tmin;
+    float tmax = ray.tmax;
+    h.leaf = -1;
+    int sp = 0;
+    uint32_t cur = 0;
+    uint32_t mask = visit8<COUNT>(sc, r, 0, tmin, tmax, cnt);
+    bool done = false;
+    for (;;) {
+        if (mask == 0) {
+            if (sp == 0) break;
+            const uint32_t e = lds_stack[(--sp) * lane_stride];
+            cur = e >> 8;
+            mask = e & 0xffu;
+            continue;
+        }
+        const uint32_t i = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        const uint32_t slot = i ^ r.octant;
+        const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
+        if (ref >= 0) {
+            if (mask) lds_stack[(sp++) * lane_stride] = (cur << 8) | mask;
+            cur = (uint32_t)ref;
+            mask = visit8<COUNT>(sc, r, cur, tmin, tmax, cnt);
+        } else {
+            leaf_test<ANY, COUNT>(sc, r, ref, tmin, tmax, h, cnt, done);
+            if (ANY && done) return true;
+        }
+    }
+    return h.leaf >= 0;
+}
+
+}  // namespace ngi

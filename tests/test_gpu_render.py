@@ -1,0 +1,114 @@
+"""GPU renders against the CPU FP64 oracle.
+
+Both consume the same counter-based random numbers for a given (pixel,
+sample, vertex, stream), so the GPU image equals the oracle image path by
+path up to floating-point rounding; paths diverge only where an FP32 ray
+hits different geometry than the FP64 one (edges, grazing angles).  The
+bar, written per test: at least 98% of pixels within a relative 2e-4 (FP32)
+or 1e-6 (FP64 shading) of the oracle, and the image means within a relative
+2e-3.  Independent-seed statistical checks (per-pixel z-test) cover the
+rest.
+"""
+import numpy as np
+import pytest
+
+import nanogi_amd as ngi
+import oracle
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = [ngi.ENGINE_MEGAKERNEL]
+
+
+def _params(renderer, w, h, spp, m, rr_depth=-1, seed=1, precision=ngi.PRECISION_FP32_FP64ACC,
+            engine=ngi.ENGINE_MEGAKERNEL, sampling=ngi.SAMPLING_PER_PIXEL):
+    return ngi.default_params(renderer=renderer, width=w, height=h, spp=spp, max_num_vertices=m, rr_depth=rr_depth,
+                              seed=seed, precision=precision, engine=engine, sampling=sampling)
+
+
+def _crn_close(gpu, ref, pix_rtol, frac=0.98, mean_rtol=2e-3):
+    err = np.abs(gpu - ref).max(axis=2)
+    scale = np.abs(ref).max(axis=2) + 1e-3 * np.abs(ref).max()
+    ok = err <= pix_rtol * scale
+    assert ok.mean() >= frac, f"only {ok.mean():.4f} of pixels agree (worst {np.max(err / scale):.3g})"
+    np.testing.assert_allclose(gpu.reshape(-1, 3).mean(0), ref.reshape(-1, 3).mean(0), rtol=mean_rtol)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_cornell_common_random_numbers(engine, renderer, width):
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    s.build(width)
+    p = _params(renderer, 48, 48, 8, 9, engine=engine)
+    g, _ = ngi.render(s, p)
+    o = oracle.render(s, p)
+    _crn_close(g, o, 2e-4)
+
+
+@pytest.mark.parametrize("precision,tol", [(ngi.PRECISION_FP64_ALL, 1e-6), (ngi.PRECISION_FP32_ALL, 3e-4)])
+def test_precision_variants(precision, tol):
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params("ptdirect", 32, 32, 8, 9, rr_depth=2, precision=precision)
+    g, _ = ngi.render(s, p)
+    _crn_close(g, oracle.render(s, p), tol)
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_russian_roulette_and_reference_sampling(renderer):
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params(renderer, 32, 24, 1, -1, rr_depth=1, sampling=ngi.SAMPLING_REFERENCE_UNIFORM)
+    p.num_samples = 32 * 24 * 16
+    s2 = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 32 / 24)
+    g, st = ngi.render(s2, p)
+    assert st["samples"] == p.num_samples
+    _crn_close(g, oracle.render(s2, p), 2e-4, frac=0.97)
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_glossy_sphere_field(renderer):
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=4)
+    p = _params(renderer, 32, 32, 4, 6, rr_depth=3)
+    g, _ = ngi.render(s, p)
+    _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
+
+
+def test_yaml_scene_with_specular():
+    s = ngi.Scene.load_yaml("tests/golden/runc_data/scene.yml", 1.0)
+    p = _params("pt", 32, 32, 8, 8, rr_depth=2)
+    g, _ = ngi.render(s, p)
+    _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
+
+
+def test_nave_ptdirect():
+    s = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
+    p = _params("ptdirect", 32, 32, 2, 8, rr_depth=4)
+    g, _ = ngi.render(s, p)
+    _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
+
+
+@pytest.mark.parametrize("m", [2, 3, 6])
+def test_furnace_closed_form(m):
+    Le, rho = (1.0, 0.5, 0.25), (0.5, 0.6, 0.7)
+    s = scenes.furnace(Le, rho)
+    g, _ = ngi.render(s, _params("pt", 16, 16, 16, m))
+    expect = scenes.furnace_radiance(Le, rho, m)
+    close = np.isclose(g, expect, rtol=1e-5).all(axis=2)
+    assert close.mean() > 0.97
+    np.testing.assert_allclose(g.reshape(-1, 3).mean(0), expect, rtol=2e-3)
+
+
+def test_independent_seed_z_test():
+    """Per-pixel z-test of the GPU estimate against an independent oracle run."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params("ptdirect", 16, 16, 1, 9, rr_depth=3)
+    sess = ngi.Session(s, p)
+    for _ in range(16):
+        sess.render_batch(16)
+    g, var = sess.film(variance=True)
+    q = _params("ptdirect", 16, 16, 4096, 9, rr_depth=3, seed=977)
+    o = oracle.render(s, q)
+    z = (g - o) / np.sqrt(var + 1e-12)
+    frac = (np.abs(z) > 3).mean()
+    assert frac < 0.03, f"{frac:.4f} of pixel channels have |z| > 3"

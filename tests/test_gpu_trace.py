@@ -1,0 +1,119 @@
+"""GPU ray queries (ngi_trace) against the FP64 oracle intersector.
+
+Parity bar: identical triangle ids except at ties (hits within a relative
+1e-5 of each other in t, i.e. shared edges/vertices where either triangle is
+a correct answer; Embree's own tie-breaking is not pinned by the reference),
+identical hit/miss except for grazing rays (<= 0.05%), and t within
+relative 1e-5 (FP32 traversal vs FP64).
+"""
+import numpy as np
+import pytest
+
+import nanogi_amd as ngi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+T_RTOL = 1e-5
+
+
+def _random_rays(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return ngi.make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+def _bounds(scene):
+    p = scene.desc()["positions"]
+    return p.min(0), p.max(0)
+
+
+def _compare(scene, rays, use_bvh, any_hit=False):
+    g = scene.trace(rays, any_hit=any_hit)
+    o = oracle.trace(scene, rays, any_hit=any_hit, use_bvh=use_bvh)
+    gh, oh = g["tri"] != ngi.MISS, o["tri"] != ngi.MISS
+    n = len(rays)
+    mismatch = (gh != oh).sum()
+    assert mismatch <= max(1, int(5e-4 * n)), f"hit/miss disagreement on {mismatch}/{n} rays"
+    if any_hit:
+        return g, o
+    both = gh & oh
+    tg, to = g["t"][both].astype(np.float64), o["t"][both].astype(np.float64)
+    close = np.abs(tg - to) <= T_RTOL * np.maximum(1.0, np.abs(to))
+    assert close.mean() >= 0.9995, f"t mismatch on {(~close).sum()} rays"
+    same = g["tri"][both] == o["tri"][both]
+    # Differing ids are only allowed where the two hits coincide (a tie).
+    assert np.all(same | close), "different triangle at a different distance"
+    assert same.mean() >= 0.995, f"triangle id agreement {same.mean():.5f}"
+    return g, o
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_trace_cornell_brute_force(width):
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    s.build(width)
+    rays = _random_rays([0.01, 0.01, 0.01], [0.99, 0.99, 2.0], 20000, 1)
+    g, o = _compare(s, rays, use_bvh=False)
+    assert (g["tri"] != ngi.MISS).mean() > 0.5
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_trace_yaml_scene(width):
+    s = ngi.Scene.load_yaml("tests/golden/runc_data/scene.yml", 1.0)
+    s.build(width)
+    lo, hi = _bounds(s)
+    _compare(s, _random_rays(lo, hi, 20000, 2), use_bvh=False)
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_trace_nave_against_oracle_bvh(width):
+    s = ngi.Scene.generate(ngi.GEN_NAVE, 3, 1.0)
+    s.build(width)
+    lo, hi = _bounds(s)
+    _compare(s, _random_rays(lo + 0.01, hi - 0.01, 20000, 3), use_bvh=True)
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_any_hit_matches_occlusion(width):
+    s = ngi.Scene.load_yaml("tests/golden/runc_data/scene.yml", 1.0)
+    s.build(width)
+    lo, hi = _bounds(s)
+    rays = _random_rays(lo, hi, 20000, 4)
+    rays["tmax"] = np.random.default_rng(5).uniform(0.1, 4.0, len(rays)).astype(np.float32)
+    _compare(s, rays, use_bvh=False, any_hit=True)
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_watertight_closed_sphere(width):
+    """Rays from inside a closed mesh through its vertices and edge midpoints
+    (shared by 2..6 triangles) must never miss (Woop et al. 2013)."""
+    s = ngi.Scene.load_yaml("tests/golden/runc_data/scene.yml", 1.0)
+    s.build(width)
+    d = s.desc()
+    prims = d["primitives"]
+    # Icosphere.obj (diffuse primitive index 2): closed, centre (0, 1, 0).
+    p = prims[2]
+    tris = d["faces"][p["tri_begin"]:p["tri_begin"] + p["tri_count"]]
+    P = d["positions"]
+    targets = [P[tris[:, k]] for k in range(3)]
+    targets += [0.5 * (P[tris[:, a]] + P[tris[:, b]]) for a, b in ((0, 1), (1, 2), (2, 0))]
+    tgt = np.concatenate(targets)
+    centre = P[tris.reshape(-1)].mean(0)
+    dirs = tgt - centre
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = ngi.make_rays(np.broadcast_to(centre, dirs.shape), dirs, tmin=0.0)
+    hits = s.trace(rays)
+    assert np.all(hits["tri"] != ngi.MISS)
+
+
+def test_empty_and_degenerate_inputs():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    assert len(s.trace(np.zeros(0, dtype=ngi.RAY_DTYPE))) == 0
+    rays = ngi.make_rays([[0.5, 0.5, 0.5]] * 3, [[0, 0, -1], [0, -1, 0], [1, 0, 0]])
+    rays["tmax"][1] = 0.0   # empty interval
+    h = s.trace(rays)
+    assert h["tri"][0] != ngi.MISS and h["tri"][1] == ngi.MISS and h["tri"][2] != ngi.MISS
+    # Axis-aligned rays (zero direction components) hit the walls at exact distances.
+    assert h["t"][0] == pytest.approx(0.5, rel=1e-6) and h["t"][2] == pytest.approx(0.5, rel=1e-6)

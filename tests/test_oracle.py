@@ -1,0 +1,228 @@
+"""Pins the CPU FP64 oracle against closed forms, invariants and known answers.
+
+The reference ships no tests, golden images or vectors for this path, so
+every oracle function is pinned here by mathematics that a plausible mistake
+(dropped term, wrong sign or index, missing Jacobian) would break.
+"""
+import numpy as np
+import pytest
+
+import nanogi_amd as ngi
+import oracle
+from tests import scenes
+
+# Random123 known-answer vectors for philox4x32-10 (Salmon et al. 2011).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert tuple(int(x) for x in oracle.philox(ctr, key)) == expect
+
+
+def _stat_close(run, expect, k=6, z=4.5, floor=1e-6):
+    """Mean over k independent seeds within z standard errors of `expect`."""
+    runs = np.array([run(seed) for seed in range(1, k + 1)])
+    mean = runs.mean(0)
+    se = runs.std(0, ddof=1) / np.sqrt(k)
+    assert np.all(np.abs(mean - expect) <= z * se + floor * np.abs(expect) + 1e-12), (mean, expect, se)
+    return mean, se
+
+
+def _params(renderer, w, h, spp, m, rr_depth=-1, seed=1, sampling=ngi.SAMPLING_PER_PIXEL):
+    return ngi.default_params(renderer=renderer, width=w, height=h, spp=spp, max_num_vertices=m,
+                              rr_depth=rr_depth, seed=seed, sampling=sampling)
+
+
+@pytest.mark.parametrize("m", [2, 3, 5])
+def test_furnace_pt_exact(m):
+    """pt in a closed emissive box: every path contributes Le*sum rho^k.
+
+    Exact except for paths that leave a wall within 1e-4 of a corner: the
+    adjacent wall is then closer than the absolute ray epsilon tnear = 1e-4
+    (basic.hpp:85, rt.hpp:2248) and the path escapes, as in the reference."""
+    Le, rho = (1.0, 0.5, 0.25), (0.5, 0.6, 0.7)
+    s = scenes.furnace(Le, rho)
+    film = oracle.render(s, _params("pt", 8, 8, 4, m))
+    expect = np.broadcast_to(scenes.furnace_radiance(Le, rho, m), film.shape)
+    close = np.isclose(film, expect, rtol=1e-12).all(axis=2)
+    assert close.mean() > 0.97
+    np.testing.assert_allclose(film.reshape(-1, 3).mean(0), expect[0, 0], rtol=2e-3)
+
+
+@pytest.mark.parametrize("m", [2, 4])
+def test_furnace_ptdirect_statistical(m):
+    Le, rho = (1.0, 0.5, 0.25), (0.5, 0.6, 0.7)
+    s = scenes.furnace(Le, rho)
+    expect = scenes.furnace_radiance(Le, rho, m)
+    _, se = _stat_close(lambda k: oracle.render(s, _params("ptdirect", 8, 8, 512, m, seed=k)).reshape(-1, 3).mean(0), expect)
+    assert np.all(se < 0.02 * expect)
+
+
+def test_furnace_with_russian_roulette_unbiased():
+    Le, rho = (1.0, 1.0, 1.0), (0.5, 0.5, 0.5)
+    s = scenes.furnace(Le, rho)
+    _stat_close(lambda k: oracle.render(s, _params("pt", 8, 8, 256, 6, rr_depth=1, seed=k)).reshape(-1, 3).mean(0),
+                scenes.furnace_radiance(Le, rho, 6))
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_plane_under_rectangular_light(renderer):
+    """One bounce (-m 3): radiance rho * Le * F(point -> rectangle)."""
+    a, b, h, Le, rho = 0.5, 0.3, 1.0, np.array([4.0, 3.0, 2.0]), np.array([0.8, 0.5, 0.3])
+    s = scenes.plane_under_light(a, b, h, tuple(Le), tuple(rho))
+    spp = 2048 if renderer == "ptdirect" else 16384
+    expect = rho * Le * scenes.rect_form_factor(a, b, h)
+    _, se = _stat_close(lambda k: oracle.render(s, _params(renderer, 2, 2, spp, 3, seed=k)).reshape(-1, 3).mean(0), expect)
+    assert np.all(se < 0.03 * expect)
+
+
+def test_directly_viewed_light_pt_and_splat():
+    """-m 2: pt sees Le inside the light's footprint; ptdirect's camera-vertex
+    NEE splat has the same expectation (src/nanogi.cpp:694-707)."""
+    Le = np.array([3.0, 2.0, 1.0])
+    s = scenes.light_in_view(tuple(Le))
+    pt = oracle.render(s, _params("pt", 32, 32, 16, 2))
+    frac = (0.5 / (3.0 * np.tan(np.radians(20.0)))) ** 2
+    np.testing.assert_allclose(pt.reshape(-1, 3).mean(0), Le * frac, rtol=0.01)
+    _stat_close(lambda k: oracle.render(s, _params("ptdirect", 16, 16, 64, 2, seed=k)).reshape(-1, 3).mean(0), Le * frac)
+    # Pixels fully inside the footprint see exactly Le with pt.
+    np.testing.assert_allclose(pt[16, 16], Le, rtol=1e-12)
+
+
+def test_pt_and_ptdirect_agree_on_cornell():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    means = {}
+    for r in ("pt", "ptdirect"):
+        runs = [oracle.render(s, _params(r, 8, 8, 1024, 9, seed=k)).reshape(-1, 3).mean(0) for k in range(1, 5)]
+        means[r] = (np.mean(runs, 0), np.std(runs, 0, ddof=1) / 2)
+    d = means["pt"][0] - means["ptdirect"][0]
+    se = np.hypot(means["pt"][1], means["ptdirect"][1])
+    assert np.all(np.abs(d) < 4 * se + 1e-4), (d, se)
+
+
+def test_reference_uniform_matches_per_pixel_expectation():
+    """Raster uniform over the image with W*H/N scaling (src/nanogi.cpp:434-437)
+    has the per-pixel estimator's expectation."""
+    Le, rho = (1.0, 1.0, 1.0), (0.5, 0.5, 0.5)
+    s = scenes.furnace(Le, rho)
+    p = _params("pt", 4, 4, 1, 4, sampling=ngi.SAMPLING_REFERENCE_UNIFORM)
+    p.num_samples = 4096
+    film = oracle.render(s, p)
+    # Each path contributes the same value; only corner escapes (see above) lose some.
+    np.testing.assert_allclose(film.reshape(-1, 3).mean(0), scenes.furnace_radiance(Le, rho, 4), rtol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# BSDF sampling versus evaluation (rt.hpp:692-1336)
+# ---------------------------------------------------------------------------
+def _glossy(alpha, eta=(1.657, 0.880, 0.521), k=(9.224, 6.270, 4.837), R=(1, 1, 1)):
+    return ngi.make_primitive(type=ngi.PRIM_G, G_R=R, G_eta=eta, G_k=k, G_roughness=alpha)
+
+
+def _conductor_normal_incidence(eta, k):
+    eta, k = np.asarray(eta), np.asarray(k)
+    return ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.2, 0.5])
+def test_conductor_fresnel_at_normal_incidence(alpha):
+    """fs(z, z) = R D(z) G F(1) / 4 with D(z) = 1/(pi alpha^2), G = 1."""
+    prim = _glossy(alpha)
+    _, _, fs = oracle.bsdf(prim, ngi.PRIM_G, (0, 0, 1), wo=(0, 0, 1))
+    F = fs * 4.0 * np.pi * alpha ** 2
+    np.testing.assert_allclose(F, _conductor_normal_incidence(prim["G_eta"], prim["G_k"]), rtol=1e-12)
+
+
+def test_dielectric_fresnel_at_normal_incidence():
+    prim = ngi.make_primitive(type=ngi.PRIM_S, spec_type=ngi.SPEC_FRESNEL, S_R=(1, 1, 1), S_eta1=1.0, S_eta2=1.5)
+    _, pdf, fs = oracle.bsdf(prim, ngi.PRIM_S, (0, 0, 1), wo=(0, 0, 1))
+    np.testing.assert_allclose(pdf, ((1.0 - 1.5) / (1.0 + 1.5)) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(fs, pdf, rtol=1e-12)
+
+
+@pytest.mark.parametrize("alpha,theta_i", [(0.1, 10.0), (0.3, 45.0), (0.6, 70.0)])
+def test_beckmann_sampling_matches_pdf(alpha, theta_i):
+    """Acceptance rate of the sampler == integral of its projected-solid-angle
+    pdf over the hemisphere (a missing Jacobian or wrong D fails this)."""
+    rng = np.random.default_rng(7)
+    prim = _glossy(alpha)
+    ti = np.radians(theta_i)
+    wi = (np.sin(ti), 0.0, np.cos(ti))
+    n = 20000
+    acc = 0
+    for u in rng.random((n, 3)):
+        wo, pdf, _ = oracle.bsdf(prim, ngi.PRIM_G, wi, u=u)
+        acc += wo[2] > 0
+    # Integral of pdf_proj * cos over the hemisphere by cosine-weighted sampling:
+    # E[pdf_proj * cos / (cos / pi)] = pi * E[pdf_proj].
+    tot = 0.0
+    for u in rng.random((n, 2)):
+        r, ph = np.sqrt(u[0]), 2 * np.pi * u[1]
+        wo = (r * np.cos(ph), r * np.sin(ph), np.sqrt(max(0.0, 1 - u[0])))
+        _, pdf, _ = oracle.bsdf(prim, ngi.PRIM_G, wi, wo=wo)
+        tot += np.pi * pdf
+    assert abs(acc / n - tot / n) < 0.03, (acc / n, tot / n)
+
+
+def test_lambert_sampling_is_cosine_weighted():
+    rng = np.random.default_rng(3)
+    prim = ngi.make_primitive(type=ngi.PRIM_D, D_R=(0.5, 0.5, 0.5))
+    zs = []
+    for u in rng.random((20000, 3)):
+        wo, pdf, fs = oracle.bsdf(prim, ngi.PRIM_D, (0.3, 0.1, 0.9), u=u)
+        assert pdf == pytest.approx(1 / np.pi)
+        zs.append(wo[2])
+        np.testing.assert_allclose(fs, 0.5 / np.pi)
+    assert np.mean(zs) == pytest.approx(2.0 / 3.0, abs=0.01)   # E[cos] under cos/pi
+
+
+def test_fresnel_branch_frequency():
+    prim = ngi.make_primitive(type=ngi.PRIM_S, spec_type=ngi.SPEC_FRESNEL, S_R=(1, 1, 1), S_eta1=1.0, S_eta2=1.5)
+    wi = (np.sin(1.0), 0, np.cos(1.0))
+    rng = np.random.default_rng(5)
+    refl, Fr = 0, None
+    for u in rng.random((4000, 3)):
+        wo, pdf, _ = oracle.bsdf(prim, ngi.PRIM_S, wi, u=u)
+        if wo[2] > 0:
+            refl += 1
+            Fr = pdf
+    assert abs(refl / 4000 - Fr) < 0.03
+
+
+# ---------------------------------------------------------------------------
+# Intersection
+# ---------------------------------------------------------------------------
+def test_oracle_bvh_matches_brute_force():
+    s = ngi.Scene.load_yaml("tests/golden/runc_data/scene.yml", 1.0)
+    rng = np.random.default_rng(11)
+    o = rng.uniform(-3, 3, (2000, 3)) + np.array([0, 1, 0])
+    d = rng.normal(size=(2000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = ngi.make_rays(o, d)
+    a = oracle.trace(s, rays, use_bvh=False)
+    b = oracle.trace(s, rays, use_bvh=True)
+    assert np.array_equal(a["tri"], b["tri"])
+    hit = a["tri"] != ngi.MISS
+    assert hit.sum() > 200
+    np.testing.assert_allclose(a["t"][hit], b["t"][hit], rtol=1e-6)
+
+
+def test_oracle_hit_distance_closed_form():
+    B = scenes.SceneBuilder()
+    p = B.prim(type=ngi.PRIM_D, D_R=(1, 1, 1))
+    B.rect(p, (0, 0, -2.0), (0, 0, 1), (1, 0, 0), 1.0, 1.0)
+    B.camera((0, 0, 0), (0, 0, -1))
+    s = B.scene()
+    rays = ngi.make_rays([[0.1, 0.2, 0.0], [0.1, 0.2, 0.0], [5, 5, 0]], [[0, 0, -1], [0, 0, 1], [0, 0, -1]])
+    h = oracle.trace(s, rays, use_bvh=False)
+    assert h["tri"][0] != ngi.MISS and h["t"][0] == pytest.approx(2.0)
+    assert h["tri"][1] == ngi.MISS and h["tri"][2] == ngi.MISS
+    # tmax excludes hits at or beyond it (Scene::Visible range, rt.hpp:2258).
+    rays["tmax"][0] = 2.0
+    assert oracle.trace(s, rays[:1], use_bvh=False)["tri"][0] == ngi.MISS
