@@ -92,3 +92,33 @@ def bsdf(prim, type_bits, wi, u=(0.0, 0.0, 0.0), wo=None, force_degenerated=True
     L.oracle_bsdf(p.ctypes.data, int(type_bits), wi_a.ctypes.data, u_a.ctypes.data, wo_a.ctypes.data,
                   int(wo is not None), int(force_degenerated), out.ctypes.data)
     return out[:3], out[3], out[4:]
+
+
+class LoadedScene:
+    """Oracle scene with its BVH built once (keeps the source scene alive)."""
+
+    def __init__(self, scene):
+        L = lib()
+        L.oracle_scene_new.argtypes = [ctypes.c_void_p]
+        L.oracle_scene_new.restype = ctypes.c_void_p
+        L.oracle_scene_free.argtypes = [ctypes.c_void_p]
+        L.oracle_scene_render_paths.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                                ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_scene_render_paths.restype = ctypes.c_int
+        self.scene = scene
+        self.desc = scene.raw_desc()
+        self.h = L.oracle_scene_new(ctypes.addressof(self.desc))
+        if not self.h:
+            raise RuntimeError("oracle could not load the scene")
+
+    def render_paths(self, params, path_begin, path_count, num_threads=0):
+        film = np.zeros(params.width * params.height * 3, dtype=np.float64)
+        counts = np.zeros(2, dtype=np.int64)
+        _check(lib().oracle_scene_render_paths(self.h, ctypes.addressof(params), int(path_begin), int(path_count),
+                                               film.ctypes.data, int(num_threads), counts.ctypes.data))
+        return film.reshape(params.height, params.width, 3), counts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_scene_free(self.h)
+            self.h = None

@@ -820,3 +820,50 @@ void oracle_bsdf(const ngi_primitive* prim, int type, const double wi[3], const 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Loaded oracle scene (BVH built once; the desc arrays must outlive it).
+void* oracle_scene_new(const ngi_scene_desc* d) {
+    Scene* s = new Scene();
+    if (load(*s, d, true)) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void oracle_scene_free(void* h) { delete reinterpret_cast<Scene*>(h); }
+
+// As oracle_render_paths, on a loaded scene: only the sample loop is timed
+// by callers (the region the reference times, src/nanogi.cpp:201-216).
+int oracle_scene_render_paths(void* h, const ngi_render_params* prm, uint64_t path_begin, uint64_t path_count, double* film,
+                              int num_threads, long long* counts) {
+    const Scene& s = *reinterpret_cast<Scene*>(h);
+    if (s.sensor < 0 || s.prims[s.sensor].p.sensor_type != NGI_SENSOR_PINHOLE) return NGI_ERR_UNSUPPORTED;
+    const size_t np = (size_t)prm->width * prm->height * 3;
+    int T = num_threads > 0 ? num_threads : (int)std::thread::hardware_concurrency();
+    T = std::max(1, std::min<int>(T, (int)std::max<uint64_t>(1, path_count / 64)));
+    std::vector<std::vector<double>> films(T, std::vector<double>(np, 0.0));
+    std::vector<Counts> cnts(T);
+    std::atomic<uint64_t> next(0);
+    auto work = [&](int k) {
+        for (;;) {
+            const uint64_t b = next.fetch_add(256);
+            if (b >= path_count) break;
+            const uint64_t e = std::min(path_count, b + 256);
+            for (uint64_t i = b; i < e; ++i) process_sample(s, *prm, path_begin + i, films[k], cnts[k]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < T; ++k) th.emplace_back(work, k);
+    work(0);
+    for (auto& t : th) t.join();
+    for (int k = 0; k < T; ++k) {
+        for (size_t i = 0; i < np; ++i) film[i] += films[k][i];
+        if (counts) counts[0] += cnts[k].ext, counts[1] += cnts[k].shadow;
+    }
+    return NGI_OK;
+}
+
+}  // extern "C"

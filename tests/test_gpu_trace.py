@@ -46,7 +46,8 @@ def _compare(scene, rays, use_bvh, any_hit=False):
     same = g["tri"][both] == o["tri"][both]
     # Differing ids are only allowed where the two hits coincide (a tie).
     assert np.all(same | close), "different triangle at a different distance"
-    assert same.mean() >= 0.995, f"triangle id agreement {same.mean():.5f}"
+    # Coplanar overlaps (box bottoms on the floor) make genuine ties common.
+    assert same.mean() >= 0.98, f"triangle id agreement {same.mean():.5f}"
     return g, o
 
 
@@ -111,7 +112,8 @@ def test_watertight_closed_sphere(width):
 def test_empty_and_degenerate_inputs():
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
     assert len(s.trace(np.zeros(0, dtype=ngi.RAY_DTYPE))) == 0
-    rays = ngi.make_rays([[0.5, 0.5, 0.5]] * 3, [[0, 0, -1], [0, -1, 0], [1, 0, 0]])
+    # From above the boxes (tops at y <= 0.6): back wall and right wall at 0.5.
+    rays = ngi.make_rays([[0.5, 0.9, 0.5]] * 3, [[0, 0, -1], [0, -1, 0], [1, 0, 0]])
     rays["tmax"][1] = 0.0   # empty interval
     h = s.trace(rays)
     assert h["tri"][0] != ngi.MISS and h["tri"][1] == ngi.MISS and h["tri"][2] != ngi.MISS
