@@ -18,7 +18,7 @@ from tests import scenes
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = [ngi.ENGINE_MEGAKERNEL]
+ENGINES = [ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_WAVEFRONT]
 
 
 def _params(renderer, w, h, spp, m, rr_depth=-1, seed=1, precision=ngi.PRECISION_FP32_FP64ACC,
@@ -58,10 +58,10 @@ def test_precision_variants(precision, tol):
     _crn_close(g, oracle.render(s, p), tol)
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
-def test_russian_roulette_and_reference_sampling(renderer):
-    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
-    p = _params(renderer, 32, 24, 1, -1, rr_depth=1, sampling=ngi.SAMPLING_REFERENCE_UNIFORM)
+def test_russian_roulette_and_reference_sampling(engine, renderer):
+    p = _params(renderer, 32, 24, 1, -1, rr_depth=1, sampling=ngi.SAMPLING_REFERENCE_UNIFORM, engine=engine)
     p.num_samples = 32 * 24 * 16
     s2 = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 32 / 24)
     g, st = ngi.render(s2, p)
@@ -69,10 +69,11 @@ def test_russian_roulette_and_reference_sampling(renderer):
     _crn_close(g, oracle.render(s2, p), 2e-4, frac=0.97)
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
-def test_glossy_sphere_field(renderer):
+def test_glossy_sphere_field(engine, renderer):
     s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=4)
-    p = _params(renderer, 32, 32, 4, 6, rr_depth=3)
+    p = _params(renderer, 32, 32, 4, 6, rr_depth=3, engine=engine)
     g, _ = ngi.render(s, p)
     _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
 
@@ -84,18 +85,20 @@ def test_yaml_scene_with_specular():
     _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
 
 
-def test_nave_ptdirect():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_nave_ptdirect(engine):
     s = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
-    p = _params("ptdirect", 32, 32, 2, 8, rr_depth=4)
+    p = _params("ptdirect", 32, 32, 2, 8, rr_depth=4, engine=engine)
     g, _ = ngi.render(s, p)
     _crn_close(g, oracle.render(s, p), 5e-4, frac=0.97, mean_rtol=5e-3)
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("m", [2, 3, 6])
-def test_furnace_closed_form(m):
+def test_furnace_closed_form(engine, m):
     Le, rho = (1.0, 0.5, 0.25), (0.5, 0.6, 0.7)
     s = scenes.furnace(Le, rho)
-    g, _ = ngi.render(s, _params("pt", 16, 16, 16, m))
+    g, _ = ngi.render(s, _params("pt", 16, 16, 16, m, engine=engine))
     expect = scenes.furnace_radiance(Le, rho, m)
     close = np.isclose(g, expect, rtol=1e-5).all(axis=2)
     assert close.mean() > 0.97
@@ -115,3 +118,37 @@ def test_independent_seed_z_test():
     z = (g - o) / np.sqrt(var + 1e-12)
     frac = (np.abs(z) > 3).mean()
     assert frac < 0.03, f"{frac:.4f} of pixel channels have |z| > 3"
+
+
+def test_wavefront_small_queue_many_iterations():
+    """A queue far smaller than the path count forces refills every
+    iteration; the image must not depend on the queue capacity."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params("ptdirect", 24, 24, 4, 9, rr_depth=2, engine=ngi.ENGINE_WAVEFRONT)
+    p.paths_in_flight = 300
+    g, st = ngi.render(s, p)
+    assert st["samples"] == 24 * 24 * 4
+    p2 = _params("ptdirect", 24, 24, 4, 9, rr_depth=2, engine=ngi.ENGINE_MEGAKERNEL)
+    m, st2 = ngi.render(s, p2)
+    # Same paths, same random numbers; the two engines differ only in FP32
+    # instruction selection, so a handful of paths may diverge.
+    assert abs(st2["extension_rays"] - st["extension_rays"]) <= 1e-3 * st["extension_rays"]
+    _crn_close(g, m, 2e-4)
+
+
+def test_wavefront_rejects_fp64_shading():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params("pt", 8, 8, 1, 4, precision=ngi.PRECISION_FP64_ALL, engine=ngi.ENGINE_WAVEFRONT)
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.render(s, p)
+    assert e.value.status == ngi.ERR_UNSUPPORTED
+
+
+def test_session_batches_accumulate():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _params("pt", 16, 16, 1, 6, rr_depth=2, engine=ngi.ENGINE_WAVEFRONT)
+    one, _ = ngi.render(s, _params("pt", 16, 16, 4, 6, rr_depth=2, engine=ngi.ENGINE_WAVEFRONT))
+    sess = ngi.Session(s, p)
+    for _ in range(4):
+        sess.render(1)
+    np.testing.assert_allclose(sess.film(), one, rtol=1e-9, atol=1e-12)
