@@ -43,11 +43,24 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-counters", action="store_true")
+    ap.add_argument("--paths-in-flight", type=int, default=0, help="wavefront queue capacity (0 = auto)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
 def main():
     a = parse()
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4}))
+        return 0
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (multi-GPU sharding is a design "
               f"note, not part of this build)", file=sys.stderr)
@@ -73,7 +86,8 @@ def main():
 
     def params(**kw):
         base = dict(renderer="ptdirect", width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=a.spp_per_step,
-                    max_num_vertices=33, rr_depth=4, rr_prob=0.5, seed=1, precision=precision, engine=engine)
+                    max_num_vertices=33, rr_depth=4, rr_prob=0.5, seed=1, precision=precision, engine=engine,
+                    paths_in_flight=a.paths_in_flight)
         base.update(kw)
         return ngi.default_params(**base)
 

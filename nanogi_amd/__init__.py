@@ -389,7 +389,8 @@ class Session:
 def render(scene, params=None, **kw):
     """Render and return (film[H, W, 3] float64, stats dict); row 0 = bottom."""
     p = params if params is not None else default_params(**kw)
-    film = np.zeros(p.width * p.height * 3, dtype=np.float64)
+    # The library validates the size; a bad one must not fail here first.
+    film = np.zeros(max(p.width * p.height * 3, 1), dtype=np.float64)
     st = Stats()
     _check(lib().ngi_render(scene.handle, ctypes.byref(p), _ptr(film), ctypes.byref(st)))
-    return film.reshape(p.height, p.width, 3), st.as_dict()
+    return film[:p.width * p.height * 3].reshape(p.height, p.width, 3), st.as_dict()

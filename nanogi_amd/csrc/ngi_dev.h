@@ -71,6 +71,7 @@ struct DevScene {
     const double* cdf_d = nullptr;
     int num_lights = 0;
     int bvh_width = 2;
+    uint32_t stack_entries = 0;   // LDS traversal-stack entries per lane (wide BVH depth + 1)
     uint32_t num_tris = 0;
     int sensor_prim = -1;
     DevCamera cam;

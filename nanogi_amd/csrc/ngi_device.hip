@@ -133,6 +133,7 @@ int device_prepare(HostScene& s) {
     const size_t nt = s.faces.size() / 3;
     DevScene& v = d->view;
     v.bvh_width = s.bvh.width;
+    v.stack_entries = s.bvh.width == 8 ? s.bvh.max_depth + 1 : 0;
     v.num_tris = (uint32_t)nt;
     v.sensor_prim = s.sensor_prim;
     // Geometry in leaf order.
@@ -228,7 +229,7 @@ int ensure_stage(DeviceScene& d, size_t bytes) {
 template <int WIDTH, bool ANY>
 __global__ void __launch_bounds__(NGI_TRACE_BLOCK) trace_kernel(DevScene sc, const ngi_ray* __restrict__ rays, size_t n,
                                                                 ngi_hit* __restrict__ hits) {
-    __shared__ uint32_t stk[WIDTH == 8 ? NGI_BVH8_STACK * NGI_TRACE_BLOCK : 1];
+    extern __shared__ uint32_t stk[];   // sc.stack_entries x block (wide BVH only)
     const size_t i = (size_t)blockIdx.x * NGI_TRACE_BLOCK + threadIdx.x;
     if (i >= n) return;
     const ngi_ray r = rays[i];
@@ -261,8 +262,9 @@ int launch_trace(HostScene& s, const ngi_ray* d_rays, size_t n, ngi_hit* d_hits,
     }
     const unsigned grid = (unsigned)((n + NGI_TRACE_BLOCK - 1) / NGI_TRACE_BLOCK);
     if (d.view.bvh_width == 8) {
-        if (any_hit) trace_kernel<8, true><<<grid, NGI_TRACE_BLOCK, 0, d.stream>>>(d.view, d_rays, n, d_hits);
-        else trace_kernel<8, false><<<grid, NGI_TRACE_BLOCK, 0, d.stream>>>(d.view, d_rays, n, d_hits);
+        const size_t lds = (size_t)d.view.stack_entries * NGI_TRACE_BLOCK * 4;
+        if (any_hit) trace_kernel<8, true><<<grid, NGI_TRACE_BLOCK, lds, d.stream>>>(d.view, d_rays, n, d_hits);
+        else trace_kernel<8, false><<<grid, NGI_TRACE_BLOCK, lds, d.stream>>>(d.view, d_rays, n, d_hits);
     } else {
         if (any_hit) trace_kernel<2, true><<<grid, NGI_TRACE_BLOCK, 0, d.stream>>>(d.view, d_rays, n, d_hits);
         else trace_kernel<2, false><<<grid, NGI_TRACE_BLOCK, 0, d.stream>>>(d.view, d_rays, n, d_hits);

@@ -418,7 +418,9 @@ int gen_sphere_field(HostScene& s, uint64_t seed, int levels) {
     // Heightfield plane.
     B.begin(diffuse(0.5, 0.5, 0.5));
     {
-        const int nv = 512;
+        // 512^2 vertices at full detail; reduced sphere detail (test scenes)
+        // uses a 64^2 grid over the same extent.
+        const int nv = levels >= 8 ? 512 : 64;
         const double ext = 6.0;
         auto h = [&](double x, double z) { return 0.05 * noise.fbm(0.8 * x + 101.0, 0.0, 0.8 * z + 37.0, 2); };
         std::vector<d3> P((size_t)nv * nv);

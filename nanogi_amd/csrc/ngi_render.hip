@@ -27,7 +27,7 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, T v) {
 // One lane = one path sample (ProcessSample_PT / ProcessSample_PTDirect).
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
-    __shared__ uint32_t stk[WIDTH == 8 ? NGI_BVH8_STACK * NGI_MK_BLOCK : 1];
+    extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     const uint64_t local = (uint64_t)blockIdx.x * NGI_MK_BLOCK + threadIdx.x;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     if (local < a.path_count) {
@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static void launch_mk(const RenderArgs& a, hipStream_t st) {
     const unsigned grid = (unsigned)((a.path_count + NGI_MK_BLOCK - 1) / NGI_MK_BLOCK);
-    megakernel<R, FilmT, DIRECT, WIDTH, COUNT><<<grid, NGI_MK_BLOCK, 0, st>>>(a);
+    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
+    megakernel<R, FilmT, DIRECT, WIDTH, COUNT><<<grid, NGI_MK_BLOCK, lds, st>>>(a);
 }
 
 template <typename R, typename FilmT, bool COUNT>

@@ -118,14 +118,26 @@ __device__ __forceinline__ void leaf_test(const DevScene& sc, const RayPre& r, i
                                           TraceCounters& cnt, bool& done) {
     const uint32_t v = (uint32_t)(~ref);
     const uint32_t first = v >> 4, count = (v & 15u) + 1u;
-    for (uint32_t k = 0; k < count; ++k) {
-        const uint32_t leaf = first + k;
-        const float4 a = sc.tris[3 * leaf], b = sc.tris[3 * leaf + 1], c = sc.tris[3 * leaf + 2];
-        if (COUNT) cnt.tris++;
+    // Triangles are fetched two at a time so their loads overlap.
+    for (uint32_t k = 0; k < count; k += 2) {
+        const uint32_t l0 = first + k;
+        const bool two = k + 1 < count;
+        const uint32_t l1 = two ? l0 + 1 : l0;
+        const float4 a0 = sc.tris[3 * l0], b0 = sc.tris[3 * l0 + 1], c0 = sc.tris[3 * l0 + 2];
+        const float4 a1 = sc.tris[3 * l1], b1 = sc.tris[3 * l1 + 1], c1 = sc.tris[3 * l1 + 2];
+        if (COUNT) cnt.tris += two ? 2u : 1u;
         float t, u, w;
-        if (tri_test(r, a, b, c, tmin, tmax, t, u, w)) {
+        if (tri_test(r, a0, b0, c0, tmin, tmax, t, u, w)) {
             tmax = t;
-            h.t = t, h.u = u, h.v = w, h.leaf = (int32_t)leaf;
+            h.t = t, h.u = u, h.v = w, h.leaf = (int32_t)l0;
+            if (ANY) {
+                done = true;
+                return;
+            }
+        }
+        if (two && tri_test(r, a1, b1, c1, tmin, tmax, t, u, w)) {
+            tmax = t;
+            h.t = t, h.u = u, h.v = w, h.leaf = (int32_t)l1;
             if (ANY) {
                 done = true;
                 return;
@@ -204,14 +216,29 @@ __device__ bool trace_bvh2(const DevScene& sc, const TraceRay& ray, TraceHit& h,
 
 __device__ __forceinline__ float exp2_bits(uint32_t biased) { return __uint_as_float(biased << 23); }
 
+struct Refs8 {
+    int32_t c0, c1, c2, c3, c4, c5, c6, c7;
+};
+
+__device__ __forceinline__ int32_t pick_ref(const Refs8& r, uint32_t slot) {
+    const int32_t lo = (slot & 2u) ? ((slot & 1u) ? r.c3 : r.c2) : ((slot & 1u) ? r.c1 : r.c0);
+    const int32_t hi = (slot & 2u) ? ((slot & 1u) ? r.c7 : r.c6) : ((slot & 1u) ? r.c5 : r.c4);
+    return (slot & 4u) ? hi : lo;
+}
+
+// Box test of the eight children; the child references are fetched in the
+// same batch of loads so the next step does not wait on a second load.
 template <bool COUNT>
 __device__ __forceinline__ uint32_t visit8(const DevScene& sc, const RayPre& r, uint32_t node, float tmin, float tmax,
-                                           TraceCounters& cnt) {
+                                           TraceCounters& cnt, Refs8& refs) {
     const float4* n = sc.nodes + 8 * (size_t)node;
     const float4 h0 = n[0];
     const uint4 q0 = __builtin_bit_cast(uint4, n[1]);  // qlox[8], qloy[8]
     const uint4 q1 = __builtin_bit_cast(uint4, n[2]);  // qloz[8], qhix[8]
     const uint4 q2 = __builtin_bit_cast(uint4, n[3]);  // qhiy[8], qhiz[8]
+    const int4 ra = __builtin_bit_cast(int4, n[4]), rb = __builtin_bit_cast(int4, n[5]);
+    refs.c0 = ra.x, refs.c1 = ra.y, refs.c2 = ra.z, refs.c3 = ra.w;
+    refs.c4 = rb.x, refs.c5 = rb.y, refs.c6 = rb.z, refs.c7 = rb.w;
     if (COUNT) cnt.nodes++;
     const uint32_t meta = __float_as_uint(h0.w);
     const float scx = exp2_bits(meta & 0xffu), scy = exp2_bits((meta >> 8) & 0xffu), scz = exp2_bits((meta >> 16) & 0xffu);
@@ -251,7 +278,9 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
     h.leaf = -1;
     int sp = 0;
     uint32_t cur = 0;
-    uint32_t mask = visit8<COUNT>(sc, r, 0, tmin, tmax, cnt);
+    Refs8 refs;
+    uint32_t mask = visit8<COUNT>(sc, r, 0, tmin, tmax, cnt, refs);
+    bool fresh = true;   // refs hold the child references of `cur`
     bool done = false;
     for (;;) {
         if (mask == 0) {
@@ -259,16 +288,18 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
             const uint32_t e = lds_stack[(--sp) * lane_stride];
             cur = e >> 8;
             mask = e & 0xffu;
+            fresh = false;
             continue;
         }
         const uint32_t i = __builtin_ctz(mask);
         mask &= mask - 1u;
         const uint32_t slot = i ^ r.octant;
-        const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
+        const int32_t ref = fresh ? pick_ref(refs, slot) : reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
         if (ref >= 0) {
             if (mask) lds_stack[(sp++) * lane_stride] = (cur << 8) | mask;
             cur = (uint32_t)ref;
-            mask = visit8<COUNT>(sc, r, cur, tmin, tmax, cnt);
+            mask = visit8<COUNT>(sc, r, cur, tmin, tmax, cnt, refs);
+            fresh = true;
         } else {
             leaf_test<ANY, COUNT>(sc, r, ref, tmin, tmax, h, cnt, done);
             if (ANY && done) return true;

@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_generate(RenderArgs a, uint64
 template <int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_extend(DevScene sc, const float4* __restrict__ ray, float4* __restrict__ hit, uint32_t n,
                                                          unsigned long long* counters) {
-    __shared__ uint32_t stk[WIDTH == 8 ? NGI_BVH8_STACK * NGI_WF_BLOCK : 1];
+    extern __shared__ uint32_t stk[];   // sc.stack_entries x block (wide BVH only)
     const uint32_t i = blockIdx.x * NGI_WF_BLOCK + threadIdx.x;
     TraceCounters tc{0, 0};
     if (i < n) {
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
 template <int WIDTH, typename FilmT, bool COUNT>
 __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shadow(DevScene sc, const float4* __restrict__ q, const uint32_t* counts, uint32_t bound,
                                                          void* film, unsigned long long* counters) {
-    __shared__ uint32_t stk[WIDTH == 8 ? NGI_BVH8_STACK * NGI_WF_BLOCK : 1];
+    extern __shared__ uint32_t stk[];   // sc.stack_entries x block (wide BVH only)
     const uint32_t i = blockIdx.x * NGI_WF_BLOCK + threadIdx.x;
     const uint32_t n = min(counts[1], bound);
     TraceCounters tc{0, 0};
@@ -227,6 +227,7 @@ static int run_wavefront(Session& S, WavefrontState& W, void* target, uint64_t t
     a.sample_base = S.samples_done;
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(S.counters);
     const uint32_t C = W.capacity;
+    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_WF_BLOCK * 4 : 0;
     uint64_t next = 0;          // next path of this render call
     // Path ids: per-pixel ids restart at 0 each call (sample_base carries the
     // sample offset); reference-uniform ids are global sample indices.
@@ -247,7 +248,7 @@ static int run_wavefront(Session& S, WavefrontState& W, void* target, uint64_t t
         }
         if ((rc = S.timed_end(tl))) return rc;
         if ((rc = S.timed_begin(K_EXTEND, n, tl))) return rc;
-        wf_extend<WIDTH, COUNT><<<(n + NGI_WF_BLOCK - 1) / NGI_WF_BLOCK, NGI_WF_BLOCK, 0, st>>>(a.sc, W.ray[cur], W.hit, n, counters);
+        wf_extend<WIDTH, COUNT><<<(n + NGI_WF_BLOCK - 1) / NGI_WF_BLOCK, NGI_WF_BLOCK, lds, st>>>(a.sc, W.ray[cur], W.hit, n, counters);
         if ((rc = S.timed_end(tl))) return rc;
         if ((rc = S.timed_begin(K_SHADE, n, tl))) return rc;
         wf_shade<DIRECT, FilmT><<<(n + NGI_WF_BLOCK - 1) / NGI_WF_BLOCK, NGI_WF_BLOCK, 0, st>>>(a, W.ray[cur], W.st[cur], W.hit, n, W.ray[cur ^ 1],
@@ -257,7 +258,7 @@ static int run_wavefront(Session& S, WavefrontState& W, void* target, uint64_t t
             // At most one shadow ray per generated path and per shaded path.
             const uint32_t bound = n_new + n;
             if ((rc = S.timed_begin(K_SHADOW, bound, tl))) return rc;
-            wf_shadow<WIDTH, FilmT, COUNT><<<(bound + NGI_WF_BLOCK - 1) / NGI_WF_BLOCK, NGI_WF_BLOCK, 0, st>>>(a.sc, W.shadow, W.counts, bound,
+            wf_shadow<WIDTH, FilmT, COUNT><<<(bound + NGI_WF_BLOCK - 1) / NGI_WF_BLOCK, NGI_WF_BLOCK, lds, st>>>(a.sc, W.shadow, W.counts, bound,
                                                                                                            target, counters);
             if ((rc = S.timed_end(tl))) return rc;
         }
