@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--spp-per-step", type=int, default=4)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
-    ap.add_argument("--engine", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--engine", choices=["wavefront", "megakernel", "persistent"], default="wavefront")
     ap.add_argument("--precision", choices=["fp32_fp64acc", "fp32_all", "fp64_all"], default="fp32_fp64acc")
     ap.add_argument("--bvh", type=int, choices=[2, 8], default=8)
     ap.add_argument("--detail", type=int, default=8, help="icosphere subdivision levels")
@@ -80,7 +80,7 @@ def main():
     ntris = scene.num_triangles
     log(f"scene: {ntris} triangles generated in {gen_s:.2f} s; BVH{a.bvh} {info['num_nodes']} nodes, "
         f"depth {info['max_depth']}, built in {info['build_ms'] / 1e3:.2f} s")
-    engine = ngi.ENGINE_WAVEFRONT if a.engine == "wavefront" else ngi.ENGINE_MEGAKERNEL
+    engine = {"wavefront": ngi.ENGINE_WAVEFRONT, "megakernel": ngi.ENGINE_MEGAKERNEL, "persistent": ngi.ENGINE_PERSISTENT}[a.engine]
     precision = {"fp32_fp64acc": ngi.PRECISION_FP32_FP64ACC, "fp32_all": ngi.PRECISION_FP32_ALL,
                  "fp64_all": ngi.PRECISION_FP64_ALL}[a.precision]
 

@@ -209,7 +209,9 @@ enum { NGI_SAMPLING_PER_PIXEL = 0,            /* spp stratified by pixel */
 enum { NGI_PRECISION_FP32_FP64ACC = 0,        /* FP32 traversal+shading, FP64 film */
        NGI_PRECISION_FP32_ALL = 1,            /* FP32 film too */
        NGI_PRECISION_FP64_ALL = 2 };          /* FP64 shading (traversal stays FP32, as Embree) */
-enum { NGI_ENGINE_MEGAKERNEL = 0, NGI_ENGINE_WAVEFRONT = 1 };
+enum { NGI_ENGINE_MEGAKERNEL = 0,   /* one path per lane */
+       NGI_ENGINE_WAVEFRONT = 1,    /* staged queues: generate / extend / shade / shadow */
+       NGI_ENGINE_PERSISTENT = 2 }; /* persistent lanes with path regeneration */
 
 typedef struct ngi_render_params {
     int renderer;            /* NGI_RENDERER_PT / NGI_RENDERER_PTDIRECT */

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libngi.so")
+LIB_PATH = os.environ.get("NGI_LIB_PATH") or os.path.join(_HERE, "libngi.so")
 
 c_int, c_uint32, c_int64, c_double, c_float, c_size_t = (
     ctypes.c_int, ctypes.c_uint32, ctypes.c_int64, ctypes.c_double, ctypes.c_float, ctypes.c_size_t)
@@ -33,7 +33,7 @@ BVH_2, BVH_8 = 2, 8
 RENDERER_PT, RENDERER_PTDIRECT, RENDERER_LT, RENDERER_LTDIRECT, RENDERER_BDPT, RENDERER_PTMNEE = range(6)
 SAMPLING_PER_PIXEL, SAMPLING_REFERENCE_UNIFORM = 0, 1
 PRECISION_FP32_FP64ACC, PRECISION_FP32_ALL, PRECISION_FP64_ALL = 0, 1, 2
-ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
+ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1, 2
 MISS = 0xFFFFFFFF
 
 _d3 = c_double * 3

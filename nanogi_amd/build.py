@@ -32,35 +32,36 @@ def _run(cmd):
     return r.stdout
 
 
-def _compile(src):
+def _compile(src, out_dir=OUT_DIR, extra=()):
     base = os.path.splitext(os.path.basename(src))[0]
-    obj = os.path.join(OUT_DIR, base + ".o")
+    obj = os.path.join(out_dir, base + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + list(extra) + ["-c", src, "-o", obj]
     else:
         cmd = [CXX] + HOST_FLAGS + ["-c", src, "-o", obj]
     _run(cmd)
     return obj
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose=False, jobs=None, lib_path=LIB, out_dir=OUT_DIR, extra_hip_flags=()):
+    """Compile every source; `extra_hip_flags`/`lib_path` build variants."""
+    os.makedirs(out_dir, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".cpp", ".hip")))
     lib_srcs = [s for s in srcs if not os.path.basename(s).startswith("cli_")]
     cli_srcs = [s for s in srcs if os.path.basename(s).startswith("cli_")]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, lib_srcs + cli_srcs))
+        objs = list(ex.map(lambda src: _compile(src, out_dir, extra_hip_flags), lib_srcs + cli_srcs))
     lib_objs = objs[:len(lib_srcs)]
     cli_objs = objs[len(lib_srcs):]
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + lib_objs + ["-lpthread"])
-    os.replace(tmp, LIB)
-    if cli_objs:
+    os.replace(tmp, lib_path)
+    if cli_objs and lib_path == LIB:
         _run([CXX, "-o", CLI] + cli_objs + ["-L", HERE, "-lngi", "-Wl,-rpath,$ORIGIN", "-lpthread"])
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":

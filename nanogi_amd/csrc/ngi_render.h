@@ -18,6 +18,7 @@ struct TimedLaunch {
 struct WavefrontState;   // ngi_wavefront.hip
 
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st);
+int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
 
 struct Session {
     ngi_scene* scene = nullptr;
@@ -29,6 +30,7 @@ struct Session {
     void* m1 = nullptr;          // double[npix*3] sum of normalised batch films
     void* m2 = nullptr;          // double[npix*3] sum of squares
     void* batch = nullptr;       // FilmT[npix*3] current batch film
+    void* next_path = nullptr;   // unsigned long long path counter of the persistent engine
     uint64_t paths_done = 0;
     uint32_t samples_done = 0;
     int batches = 0;

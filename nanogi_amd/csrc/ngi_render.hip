@@ -24,6 +24,105 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, T v) {
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(p, x);
 }
 
+// Per-lane path state between vertices of ProcessSample_PT/_PTDirect.
+template <typename R>
+struct PathState {
+    V3<R> T;          // throughput
+    Surf<R> s;        // current vertex
+    V3<R> wi;         // direction towards the previous vertex
+    int type, prim, nv, pixel;
+    uint32_t c0, c1;  // RNG counter words of this path
+};
+
+// Camera vertex of path `id` (SampleEmitter(E) + SamplePosition, src/nanogi.cpp:448-480).
+template <typename R>
+__device__ __forceinline__ void path_start(const RenderArgs& a, uint64_t id, const Cam<R>& cam, PathState<R>& ps) {
+    path_coords(a, id, ps.c0, ps.c1, ps.pixel);
+    ps.T = V3<R>(R(1));
+    ps.type = NGI_PRIM_E;
+    ps.prim = a.sc.sensor_prim;
+    ps.s.p = cam.eye;
+    ps.s.degenerate = true;
+    ps.s.gn = V3<R>(R(0));
+    ps.s.f.n = ps.s.f.s = ps.s.f.t = V3<R>(R(0));
+    ps.wi = V3<R>(R(0));
+    ps.nv = 1;
+}
+
+// One iteration of the vertex loop (src/nanogi.cpp:483-606 / 645-801):
+// NEE (ptdirect), direction sample, extension ray, emission (pt), Russian
+// roulette.  Returns false when the path ends.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+__device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
+                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt) {
+    const DevScene& sc = a.sc;
+    if (ps.nv >= max_nv) return false;
+    if (DIRECT) {
+        const U4 rn = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_NEE, a.seed, a.stream_id);
+        TraceRay sr;
+        int target;
+        const V3<R> C = nee_contribution<R>(sc, cam, prims, ps.type, ps.prim, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
+        if (!is_zero(C)) {
+            TraceHit sh;
+            TraceCounters tc{0, 0};
+            const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, stk, stride);
+            cnt[CNT_SHADOW_RAYS]++;
+            if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+            if (!occluded) film_add<FilmT>(a.film, target, C);
+        }
+    }
+    const U4 rx = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_EXT, a.seed, a.stream_id);
+    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
+    V3<R> wo;
+    if (ps.type == NGI_PRIM_E) {
+        wo = camera_sample<R>(a, cam, u0, u1, ps.pixel);
+    } else {
+        V3<R> wgt;
+        if (!bsdf_sample<R>(prims[ps.prim], ps.type, ps.s, ps.wi, u0, u1, uc, wo, wgt)) return false;
+        ps.T = ps.T * wgt;
+    }
+    TraceRay er;
+    er.ox = (float)ps.s.p.x, er.oy = (float)ps.s.p.y, er.oz = (float)ps.s.p.z;
+    er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
+    er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
+    TraceHit h;
+    TraceCounters tc{0, 0};
+    const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, stk, stride);
+    cnt[CNT_EXT_RAYS]++;
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    if (!hit) return false;
+    int hp;
+    const Surf<R> ns = hit_surface<R>(sc, ps.s.p, wo, h, hp);
+    const DevPrim<R>& hm = prims[hp];
+    if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
+        // Emission on hit (src/nanogi.cpp:568-575): area lights only.
+        const DevLight<R>& L = lights_of<R>(sc)[hm.light];
+        if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -wo) > R(0))
+            film_add<FilmT>(a.film, ps.pixel, ps.T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
+    }
+    if (a.rr_depth >= 0 && ps.nv >= a.rr_depth) {   // src/nanogi.cpp:583-591
+        const R q = rr_prob<R>(a);
+        if (urr > q) return false;
+        ps.T = ps.T / q;
+    }
+    ps.type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    if (ps.type == 0) return false;   // [L]-only primitives absorb (rt.hpp:1146-1147)
+    ps.s = ns;
+    ps.prim = hp;
+    ps.wi = -wo;
+    ps.nv++;
+    return true;
+}
+
+__device__ __forceinline__ int clamp_vertices(int m) { return (m < 0 || m > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : m; }
+
+template <typename T>
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, const T* cnt, bool count) {
+    if (!counters) return;
+    for (int k = 0; k < CNT_N; ++k)
+        if (count || (k != CNT_EXT_NODES && k != CNT_EXT_TRIS && k != CNT_SH_NODES && k != CNT_SH_TRIS)) wave_add(counters + k, cnt[k]);
+}
+
 // One lane = one path sample (ProcessSample_PT / ProcessSample_PTDirect).
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
@@ -31,87 +130,54 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
     const uint64_t local = (uint64_t)blockIdx.x * NGI_MK_BLOCK + threadIdx.x;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     if (local < a.path_count) {
-        const DevScene& sc = a.sc;
-        const uint64_t id = a.path_begin + local;
-        uint32_t c0, c1;
-        int pixel;
-        path_coords(a, id, c0, c1, pixel);
-        const Cam<R> cam = load_cam<R>(sc.cam);
-        const DevPrim<R>* prims = prims_of<R>(sc);
-        V3<R> T(R(1));
-        int type = NGI_PRIM_E, prim = sc.sensor_prim;
-        Surf<R> s;
-        s.p = cam.eye;
-        s.degenerate = true;
-        s.gn = V3<R>(R(0));
-        s.f.n = s.f.s = s.f.t = V3<R>(R(0));
-        V3<R> wi(R(0));
-        int nv = 1;
-        const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
+        const Cam<R> cam = load_cam<R>(a.sc.cam);
+        const DevPrim<R>* prims = prims_of<R>(a.sc);
+        PathState<R> ps;
+        path_start<R>(a, a.path_begin + local, cam, ps);
         cnt[CNT_SAMPLES] = 1;
-        uint32_t* my = stk + threadIdx.x;
-        for (;;) {
-            if (nv >= max_nv) break;
-            if (DIRECT) {
-                const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
-                TraceRay sr;
-                int target;
-                const V3<R> C = nee_contribution<R>(sc, cam, prims, type, prim, s, wi, T, rn, a.w, a.h, pixel, sr, target);
-                if (!is_zero(C)) {
-                    TraceHit sh;
-                    TraceCounters tc{0, 0};
-                    const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, my, NGI_MK_BLOCK);
-                    cnt[CNT_SHADOW_RAYS]++;
-                    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
-                    if (!occluded) film_add<FilmT>(a.film, target, C);
-                }
-            }
-            const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
-            const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
-            V3<R> wo;
-            if (type == NGI_PRIM_E) {
-                wo = camera_sample<R>(a, cam, u0, u1, pixel);
-            } else {
-                V3<R> wgt;
-                if (!bsdf_sample<R>(prims[prim], type, s, wi, u0, u1, uc, wo, wgt)) break;
-                T = T * wgt;
-            }
-            TraceRay er;
-            er.ox = (float)s.p.x, er.oy = (float)s.p.y, er.oz = (float)s.p.z;
-            er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
-            er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
-            TraceHit h;
-            TraceCounters tc{0, 0};
-            const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, my, NGI_MK_BLOCK);
-            cnt[CNT_EXT_RAYS]++;
-            if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
-            if (!hit) break;
-            int hp;
-            const Surf<R> ns = hit_surface<R>(sc, s.p, wo, h, hp);
-            const DevPrim<R>& hm = prims[hp];
-            if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
-                // Emission on hit (src/nanogi.cpp:568-575): area lights only.
-                const DevLight<R>& L = lights_of<R>(sc)[hm.light];
-                if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -wo) > R(0))
-                    film_add<FilmT>(a.film, pixel, T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
-            }
-            if (a.rr_depth >= 0 && nv >= a.rr_depth) {   // src/nanogi.cpp:583-591
-                const R q = rr_prob<R>(a);
-                if (urr > q) break;
-                T = T / q;
-            }
-            type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
-            if (type == 0) break;   // [L]-only primitives absorb (rt.hpp:1146-1147)
-            s = ns;
-            prim = hp;
-            wi = -wo;
-            nv++;
+        const int max_nv = clamp_vertices(a.max_nv);
+        while (path_vertex<R, FilmT, DIRECT, WIDTH, COUNT>(a, cam, prims, ps, stk + threadIdx.x, NGI_MK_BLOCK, max_nv, cnt)) {
         }
     }
-    if (a.counters) {
-        for (int k = 0; k < CNT_N; ++k)
-            if (COUNT || (k != CNT_EXT_NODES && k != CNT_EXT_TRIS && k != CNT_SH_NODES && k != CNT_SH_TRIS)) wave_add(a.counters + k, cnt[k]);
+    flush_counters(a.counters, cnt, COUNT);
+}
+
+// Persistent lanes with path regeneration: a lane whose path ended takes the
+// next path id (one atomic per wavefront and iteration), so every iteration
+// traces with all lanes busy regardless of how uneven path lengths are.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+__global__ void __launch_bounds__(NGI_MK_BLOCK) persistent_kernel(RenderArgs a, unsigned long long* next) {
+    extern __shared__ uint32_t stk[];
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    const Cam<R> cam = load_cam<R>(a.sc.cam);
+    const DevPrim<R>* prims = prims_of<R>(a.sc);
+    const int max_nv = clamp_vertices(a.max_nv);
+    const uint32_t lane = __lane_id();
+    PathState<R> ps;
+    bool active = false, exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= a.path_count) exhausted = true;
+                if (!active) {
+                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    if (local < a.path_count) {
+                        path_start<R>(a, a.path_begin + local, cam, ps);
+                        cnt[CNT_SAMPLES]++;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) active = path_vertex<R, FilmT, DIRECT, WIDTH, COUNT>(a, cam, prims, ps, stk + threadIdx.x, NGI_MK_BLOCK, max_nv, cnt);
     }
+    flush_counters(a.counters, cnt, COUNT);
 }
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
@@ -130,6 +196,34 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
         if (width == 8) launch_mk<R, FilmT, false, 8, COUNT>(a, st);
         else launch_mk<R, FilmT, false, 2, COUNT>(a, st);
     }
+}
+
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+static int launch_pk(const RenderArgs& a, unsigned long long* next, hipStream_t st) {
+    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
+    auto kern = persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
+    int dev = 0, cus = 0, per_cu = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_MK_BLOCK, lds));
+    const uint64_t need_blocks = (a.path_count + NGI_MK_BLOCK - 1) / NGI_MK_BLOCK;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next);
+    NGI_HIP_CHECK(hipGetLastError());
+    return NGI_OK;
+}
+
+template <typename R, typename FilmT, bool COUNT>
+static int launch_pk3(const RenderArgs& a, unsigned long long* next, bool direct, int width, hipStream_t st) {
+    if (direct) return width == 8 ? launch_pk<R, FilmT, true, 8, COUNT>(a, next, st) : launch_pk<R, FilmT, true, 2, COUNT>(a, next, st);
+    return width == 8 ? launch_pk<R, FilmT, false, 8, COUNT>(a, next, st) : launch_pk<R, FilmT, false, 2, COUNT>(a, next, st);
+}
+
+int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_pk3<double, double, false>(a, next, direct, width, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_pk3<float, float, false>(a, next, direct, width, st);
+    if (count) return launch_pk3<float, double, true>(a, next, direct, width, st);
+    return launch_pk3<float, double, false>(a, next, direct, width, st);
 }
 
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st) {
@@ -178,7 +272,8 @@ int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
     if (p.sampling != NGI_SAMPLING_PER_PIXEL && p.sampling != NGI_SAMPLING_REFERENCE_UNIFORM) return fail(NGI_ERR_INVALID_ARG, "bad sampling mode");
     if (p.sampling == NGI_SAMPLING_REFERENCE_UNIFORM && p.num_samples <= 0) return fail(NGI_ERR_INVALID_ARG, "num_samples must be positive");
     if (p.precision < 0 || p.precision > 2) return fail(NGI_ERR_INVALID_ARG, "bad precision");
-    if (p.engine != NGI_ENGINE_MEGAKERNEL && p.engine != NGI_ENGINE_WAVEFRONT) return fail(NGI_ERR_INVALID_ARG, "bad engine");
+    if (p.engine != NGI_ENGINE_MEGAKERNEL && p.engine != NGI_ENGINE_WAVEFRONT && p.engine != NGI_ENGINE_PERSISTENT)
+        return fail(NGI_ERR_INVALID_ARG, "bad engine");
     if (p.rr_depth >= 0 && !(p.rr_prob > 0.0 && p.rr_prob <= 1.0)) return fail(NGI_ERR_INVALID_ARG, "rr_prob must be in (0, 1]");
     if (p.max_num_vertices < -1) return fail(NGI_ERR_INVALID_ARG, "bad max_num_vertices");
     if (sc->sensor_prim < 0) return fail(NGI_ERR_INVALID_SCENE, "scene has no sensor (E) primitive");
@@ -210,8 +305,8 @@ int Session::reset() {
 
 void Session::destroy() {
     if (scene && scene->dev) (void)hipStreamSynchronize(stream());
-    for (void* q : {film, counters, m1, m2, batch}) if (q) (void)hipFree(q);
-    film = counters = m1 = m2 = batch = nullptr;
+    for (void* q : {film, counters, m1, m2, batch, next_path}) if (q) (void)hipFree(q);
+    film = counters = m1 = m2 = batch = next_path = nullptr;
     release_engine();
     for (auto& e : pending) {
         (void)hipEventDestroy(e.a);
@@ -287,6 +382,18 @@ int Session::render_into(void* target, int spp) {
     else total = (uint64_t)p.num_samples * (uint64_t)spp;
     if (p.engine == NGI_ENGINE_WAVEFRONT) {
         st = render_wavefront(target, total);
+    } else if (p.engine == NGI_ENGINE_PERSISTENT) {
+        if (!next_path) NGI_HIP_CHECK(hipMalloc(&next_path, sizeof(unsigned long long)));
+        if ((st = zero(next_path, 1, true))) return st;
+        RenderArgs a = args(target);
+        a.sample_base = samples_done;
+        a.path_begin = p.sampling == NGI_SAMPLING_PER_PIXEL ? 0 : paths_done;
+        a.path_count = total;
+        TimedLaunch tl;
+        if ((st = timed_begin(K_MEGA, (int64_t)total, tl))) return st;
+        st = launch_persistent(a, reinterpret_cast<unsigned long long*>(next_path), p.precision, p.renderer == NGI_RENDERER_PTDIRECT,
+                               scene->dev->view.bvh_width, p.collect_counters != 0, stream());
+        if (!st) st = timed_end(tl);
     } else {
         // Launches of at most 2^22 paths (whole pixels rows of samples).
         const uint64_t chunk = p.sampling == NGI_SAMPLING_PER_PIXEL ? (uint64_t)npix * std::max<uint64_t>(1, (1u << 22) / npix) : (1u << 22);

@@ -46,6 +46,8 @@ __device__ __forceinline__ float guard_dir(float d) {
     return fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
 }
 
+__device__ __forceinline__ float pick(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
 __device__ __forceinline__ RayPre make_pre(const TraceRay& r) {
     RayPre p;
     p.ox = r.ox, p.oy = r.oy, p.oz = r.oz;
@@ -57,21 +59,20 @@ __device__ __forceinline__ RayPre make_pre(const TraceRay& r) {
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;
     int ky = kx + 1 == 3 ? 0 : kx + 1;
-    const float d[3] = {r.dx, r.dy, r.dz};
-    if (d[kz] < 0.0f) {
+    // Component selection by value (no dynamically indexed private array).
+    const float dz = pick(r.dx, r.dy, r.dz, kz);
+    if (dz < 0.0f) {
         int t = kx;
         kx = ky;
         ky = t;
     }
     p.kx = kx, p.ky = ky, p.kz = kz;
-    p.sx = d[kx] / d[kz];
-    p.sy = d[ky] / d[kz];
-    p.sz = 1.0f / d[kz];
+    p.sx = pick(r.dx, r.dy, r.dz, kx) / dz;
+    p.sy = pick(r.dx, r.dy, r.dz, ky) / dz;
+    p.sz = 1.0f / dz;
     p.octant = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
     return p;
 }
-
-__device__ __forceinline__ float pick(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
 
 // Watertight test; on a hit inside (tmin, tmax) writes t, u, v.
 __device__ __forceinline__ bool tri_test(const RayPre& r, float4 a4, float4 b4, float4 c4, float tmin, float tmax, float& t,
@@ -111,6 +112,13 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, float4 a4, float4 b4, 
     return true;
 }
 
+#ifndef NGI_PAIR_TRIS
+#define NGI_PAIR_TRIS 1      // fetch leaf triangles two at a time
+#endif
+#ifndef NGI_PREFETCH_REFS
+#define NGI_PREFETCH_REFS 0  // fetch child references together with the node boxes (spills; slower)
+#endif
+
 #define NGI_SLAB_SCALE 1.00000036f   // 1 + 2*gamma(3), gamma(n) = n*eps/(1-n*eps), eps = 2^-24
 
 template <bool ANY, bool COUNT>
@@ -118,6 +126,23 @@ __device__ __forceinline__ void leaf_test(const DevScene& sc, const RayPre& r, i
                                           TraceCounters& cnt, bool& done) {
     const uint32_t v = (uint32_t)(~ref);
     const uint32_t first = v >> 4, count = (v & 15u) + 1u;
+#if !NGI_PAIR_TRIS
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t leaf = first + k;
+        const float4 a = sc.tris[3 * leaf], b = sc.tris[3 * leaf + 1], c = sc.tris[3 * leaf + 2];
+        if (COUNT) cnt.tris++;
+        float t, u, w;
+        if (tri_test(r, a, b, c, tmin, tmax, t, u, w)) {
+            tmax = t;
+            h.t = t, h.u = u, h.v = w, h.leaf = (int32_t)leaf;
+            if (ANY) {
+                done = true;
+                return;
+            }
+        }
+    }
+    return;
+#endif
     // Triangles are fetched two at a time so their loads overlap.
     for (uint32_t k = 0; k < count; k += 2) {
         const uint32_t l0 = first + k;
@@ -236,9 +261,13 @@ __device__ __forceinline__ uint32_t visit8(const DevScene& sc, const RayPre& r, 
     const uint4 q0 = __builtin_bit_cast(uint4, n[1]);  // qlox[8], qloy[8]
     const uint4 q1 = __builtin_bit_cast(uint4, n[2]);  // qloz[8], qhix[8]
     const uint4 q2 = __builtin_bit_cast(uint4, n[3]);  // qhiy[8], qhiz[8]
+#if NGI_PREFETCH_REFS
     const int4 ra = __builtin_bit_cast(int4, n[4]), rb = __builtin_bit_cast(int4, n[5]);
     refs.c0 = ra.x, refs.c1 = ra.y, refs.c2 = ra.z, refs.c3 = ra.w;
     refs.c4 = rb.x, refs.c5 = rb.y, refs.c6 = rb.z, refs.c7 = rb.w;
+#else
+    (void)refs;
+#endif
     if (COUNT) cnt.nodes++;
     const uint32_t meta = __float_as_uint(h0.w);
     const float scx = exp2_bits(meta & 0xffu), scy = exp2_bits((meta >> 8) & 0xffu), scz = exp2_bits((meta >> 16) & 0xffu);
@@ -294,7 +323,11 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
         const uint32_t i = __builtin_ctz(mask);
         mask &= mask - 1u;
         const uint32_t slot = i ^ r.octant;
+#if NGI_PREFETCH_REFS
         const int32_t ref = fresh ? pick_ref(refs, slot) : reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
+#else
+        const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
+#endif
         if (ref >= 0) {
             if (mask) lds_stack[(sp++) * lane_stride] = (cur << 8) | mask;
             cur = (uint32_t)ref;

@@ -26,6 +26,9 @@
 namespace ngi {
 
 #define NGI_WF_BLOCK 256
+#ifndef NGI_TRACE_WAVES
+#define NGI_TRACE_WAVES 1    // minimum waves per SIMD requested for the traversal kernels
+#endif
 
 struct WavefrontState {
     uint32_t capacity = 0;
@@ -105,7 +108,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_generate(RenderArgs a, uint64
 }
 
 template <int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_WF_BLOCK) wf_extend(DevScene sc, const float4* __restrict__ ray, float4* __restrict__ hit, uint32_t n,
+__global__ void __launch_bounds__(NGI_WF_BLOCK, NGI_TRACE_WAVES) wf_extend(DevScene sc, const float4* __restrict__ ray, float4* __restrict__ hit, uint32_t n,
                                                          unsigned long long* counters) {
     extern __shared__ uint32_t stk[];   // sc.stack_entries x block (wide BVH only)
     const uint32_t i = blockIdx.x * NGI_WF_BLOCK + threadIdx.x;
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
 }
 
 template <int WIDTH, typename FilmT, bool COUNT>
-__global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shadow(DevScene sc, const float4* __restrict__ q, const uint32_t* counts, uint32_t bound,
+__global__ void __launch_bounds__(NGI_WF_BLOCK, NGI_TRACE_WAVES) wf_shadow(DevScene sc, const float4* __restrict__ q, const uint32_t* counts, uint32_t bound,
                                                          void* film, unsigned long long* counters) {
     extern __shared__ uint32_t stk[];   // sc.stack_entries x block (wide BVH only)
     const uint32_t i = blockIdx.x * NGI_WF_BLOCK + threadIdx.x;

@@ -18,7 +18,7 @@ from tests import scenes
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = [ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_WAVEFRONT]
+ENGINES = [ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_WAVEFRONT, ngi.ENGINE_PERSISTENT]
 
 
 def _params(renderer, w, h, spp, m, rr_depth=-1, seed=1, precision=ngi.PRECISION_FP32_FP64ACC,
@@ -152,3 +152,16 @@ def test_session_batches_accumulate():
     for _ in range(4):
         sess.render(1)
     np.testing.assert_allclose(sess.film(), one, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_persistent_engine_matches_megakernel(renderer):
+    """Path regeneration changes which lane runs a path, not the path."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=4)
+    out = {}
+    for e in (ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT):
+        out[e] = ngi.render(s, _params(renderer, 40, 30, 8, 12, rr_depth=3, engine=e))
+    (g, st), (m, st2) = out[ngi.ENGINE_PERSISTENT], out[ngi.ENGINE_MEGAKERNEL]
+    assert st["samples"] == st2["samples"] == 40 * 30 * 8
+    assert abs(st["extension_rays"] - st2["extension_rays"]) <= 1e-3 * st2["extension_rays"]
+    _crn_close(g, m, 2e-4)
