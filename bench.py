@@ -28,6 +28,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(a, engine, precision):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
+    WRITE_SIZE x 1.0, calibrated on a 1 GiB copy) for the default workload;
+    None for other configurations."""
+    import nanogi_amd as ngi
+    path = os.path.join(ROOT, "profiles", "persistent_ptdirect_sphere_field_1024.md")
+    default = (engine == ngi.ENGINE_PERSISTENT and precision == ngi.PRECISION_FP32_FP64ACC and a.bvh == 8 and
+               a.width == 1024 and a.height == 1024 and a.spp_per_step == 4 and a.detail == 8)
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -36,7 +52,9 @@ def parse():
     ap.add_argument("--spp-per-step", type=int, default=4)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
-    ap.add_argument("--engine", choices=["wavefront", "megakernel", "persistent"], default="wavefront")
+    ap.add_argument("--engine", choices=["wavefront", "megakernel", "persistent"], default="persistent")
+    ap.add_argument("--compare-steps", type=int, default=4,
+                    help="steps for the other engines reported beside the main one (0 = skip)")
     ap.add_argument("--precision", choices=["fp32_fp64acc", "fp32_all", "fp64_all"], default="fp32_fp64acc")
     ap.add_argument("--bvh", type=int, choices=[2, 8], default=8)
     ap.add_argument("--detail", type=int, default=8, help="icosphere subdivision levels")
@@ -136,7 +154,15 @@ def main():
     node_bytes = info["node_bytes"]
     roof = None
     if counts is not None:
-        if engine == ngi.ENGINE_WAVEFRONT and st["extend_launches"] > 0:
+        if engine == ngi.ENGINE_PERSISTENT:
+            # One persistent launch per step: every path of the step.
+            avg_ms = st["ms_shade"] / a.steps
+            per_path = (counts["ext_rays_per_path"] * (node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]) +
+                        counts["shadow_rays_per_path"] * (node_bytes * counts["shadow_nodes_per_ray"] + TRI_BYTES * counts["shadow_tris_per_ray"]))
+            rays_per_launch = a.spp_per_step * npix
+            per_ray = per_path
+            kernel = "persistent path kernel (path regeneration)"
+        elif engine == ngi.ENGINE_WAVEFRONT and st["extend_launches"] > 0:
             avg_ms = st["ms_extend"] / st["extend_launches"]
             rays_per_launch = st["extend_rays_timed"] / st["extend_launches"]
             per_ray = RAY_IO_BYTES + node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]
@@ -156,9 +182,35 @@ def main():
         bytes_per_launch = per_ray * rays_per_launch
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(a, engine, precision),
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "units_per_launch": round(rays_per_launch), "bytes_per_unit": round(per_ray, 2)}
+
+    # The other engines on the same workload (short runs, same seeds).
+    engines = {}
+    if a.compare_steps > 0:
+        variants = [("megakernel_bvh8_fp32", ngi.ENGINE_MEGAKERNEL, 8, ngi.PRECISION_FP32_FP64ACC),
+                    ("wavefront_bvh8_fp32", ngi.ENGINE_WAVEFRONT, 8, ngi.PRECISION_FP32_FP64ACC),
+                    ("persistent_bvh8_fp32", ngi.ENGINE_PERSISTENT, 8, ngi.PRECISION_FP32_FP64ACC),
+                    ("megakernel_bvh2_fp64", ngi.ENGINE_MEGAKERNEL, 2, ngi.PRECISION_FP64_ALL)]
+        for name, eng, width, prec in variants:
+            if eng == engine and width == a.bvh and prec == precision:
+                engines[name] = round(value, 1)
+                continue
+            if scene.bvh_info()["width"] != width:
+                scene.build(width)
+            vs = ngi.Session(scene, params(engine=eng, precision=prec))
+            vs.render(a.spp_per_step)
+            vs.sync()
+            t1 = time.perf_counter()
+            for _ in range(a.compare_steps):
+                vs.render(a.spp_per_step)
+            vs.sync()
+            engines[name] = round(a.compare_steps * a.spp_per_step * npix / (time.perf_counter() - t1), 1)
+            vs.close()
+            log(f"{name}: {engines[name] / 1e6:.1f} M samples/s")
+        if scene.bvh_info()["width"] != a.bvh:
+            scene.build(a.bvh)
 
     # CPU baseline: the FP64 oracle on the host cores, bounded sample of paths.
     cpu = None
@@ -182,8 +234,8 @@ def main():
             _, cc = ol.render_paths(p, n, n2, threads)
             dt = time.perf_counter() - t1
             cpu = {"value": round(n2 / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
-                   "sample": f"{n2} path samples (ids {n}..{n + n2 - 1}, row-major pixels, sample 0) of the same "
-                             f"ptdirect workload; FP64 CPU restatement with its own BVH, {threads} threads",
+                   "sample": f"{n2} path samples (path ids {n}..{n + n2 - 1}; pixel = id mod W*H, sample = id div W*H) "
+                             f"of the same ptdirect workload; FP64 CPU restatement with its own BVH, {threads} threads",
                    "rays_per_s": round(float(cc.sum()) / dt, 1)}
             log(f"cpu oracle: {n2} paths in {dt:.2f} s on {threads} threads")
         except Exception as e:  # the baseline is reported, never required
@@ -203,6 +255,7 @@ def main():
         "rmse_estimate": {"spp": spp_done, "rmse": rmse_now, "rmse_at_1024spp": rmse_1024,
                           "time_to_1024spp_s": round(1024 * npix / value, 3)},
         "roofline": roof,
+        "engines_samples_per_s": engines,
         "cpu_baseline": cpu,
         "counters": counts,
         "bvh_build_s": round(info["build_ms"] / 1e3, 3),

@@ -302,7 +302,7 @@ int Session::render_wavefront(void* target, uint64_t total) {
         return fail(NGI_ERR_UNSUPPORTED, "the wavefront engine shades in FP32; FP64 shading runs on the megakernel");
     if (!wf) {
         wf = new WavefrontState();
-        uint32_t cap = p.paths_in_flight > 0 ? (uint32_t)p.paths_in_flight : (1u << 21);
+        uint32_t cap = p.paths_in_flight > 0 ? (uint32_t)p.paths_in_flight : (1u << 23);
         cap = std::max<uint32_t>(cap, 256);
         int st = wf_alloc(*wf, cap);
         if (st) return st;
