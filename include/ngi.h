@@ -279,6 +279,14 @@ int ngi_session_reset(ngi_session* s);
 void ngi_session_destroy(ngi_session* s);
 
 /* ------------------------------------------------------------------ */
+/* Film output: SaveImage (include/nanogi/basic.hpp:506-672).  Writes  */
+/* .hdr (Radiance RGBE), .exr (float B,G,R), .png (gamma 1/2.2) or     */
+/* .pfm by extension, upright; film as returned by ngi_render.        */
+/* Parent directories are created; other extensions fail.             */
+/* ------------------------------------------------------------------ */
+int ngi_image_save(const char* path, const double* film, int width, int height);
+
+/* ------------------------------------------------------------------ */
 /* Adjoint (no reference counterpart): gradients of sum(dL_dimage *    */
 /* image) with respect to per-primitive reflectance (D.R) and emitter  */
 /* radiance (L.Area.Le), by path replay from the counter-RNG seeds.   */

@@ -107,7 +107,7 @@ EXPORTS = [
     "ngi_scene_bvh_info", "ngi_scene_bvh_export", "ngi_trace", "ngi_trace_device", "ngi_renderer_from_string",
     "ngi_render_params_default", "ngi_render", "ngi_session_create", "ngi_session_render",
     "ngi_session_render_batch", "ngi_session_sync", "ngi_session_get_film", "ngi_session_get_stats",
-    "ngi_session_reset", "ngi_session_destroy", "ngi_render_adjoint", "ngi_render_batch",
+    "ngi_session_reset", "ngi_session_destroy", "ngi_render_adjoint", "ngi_render_batch", "ngi_image_save",
 ]
 
 
@@ -159,6 +159,7 @@ def lib():
         "ngi_session_destroy": ([vp], None),
         "ngi_render_adjoint": ([vp, P(RenderParams), vp, vp, vp, P(Stats)], c_int),
         "ngi_render_batch": ([vp, P(RenderParams), c_int, vp, vp, P(Stats)], c_int),
+        "ngi_image_save": ([ctypes.c_char_p, vp, c_int, c_int], c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -394,3 +395,10 @@ def render(scene, params=None, **kw):
     st = Stats()
     _check(lib().ngi_render(scene.handle, ctypes.byref(p), _ptr(film), ctypes.byref(st)))
     return film[:p.width * p.height * 3].reshape(p.height, p.width, 3), st.as_dict()
+
+
+def save_image(path, film):
+    """SaveImage (include/nanogi/basic.hpp:506): .hdr/.exr/.png/.pfm by extension."""
+    f = np.ascontiguousarray(film, dtype=np.float64)
+    h, w = f.shape[0], f.shape[1]
+    _check(lib().ngi_image_save(str(path).encode(), _ptr(f), int(w), int(h)))
