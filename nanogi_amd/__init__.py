@@ -402,3 +402,16 @@ def save_image(path, film):
     f = np.ascontiguousarray(film, dtype=np.float64)
     h, w = f.shape[0], f.shape[1]
     _check(lib().ngi_image_save(str(path).encode(), _ptr(f), int(w), int(h)))
+
+
+def render_adjoint(scene, params, dL_dimage):
+    """Path-replay gradients of sum(dL_dimage * image) (ngi_render_adjoint).
+
+    Returns (grad_reflectance, grad_Le, stats) with one row of 3 per primitive."""
+    g = np.ascontiguousarray(dL_dimage, dtype=np.float64).reshape(params.height, params.width, 3)
+    n = scene.raw_desc().num_primitives
+    gR = np.zeros((n, 3))
+    gL = np.zeros((n, 3))
+    st = Stats()
+    _check(lib().ngi_render_adjoint(scene.handle, ctypes.byref(params), _ptr(g), _ptr(gR), _ptr(gL), ctypes.byref(st)))
+    return gR, gL, st.as_dict()

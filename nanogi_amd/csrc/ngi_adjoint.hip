@@ -1,0 +1,326 @@
+// Adjoint of pt / ptdirect by path replay (no reference counterpart).
+//
+// Gradient of  L = sum_p dL/dI_p . I_p  with respect to every primitive's
+// reflectance (D.R, G.R) and every emitter's radiance (L.*.Le).
+//
+// Path sampling in ProcessSample_PT/_PTDirect (src/nanogi.cpp:446-802) does
+// not depend on these parameters: directions come from cosine / Beckmann
+// sampling, lights are picked uniformly and positions by area, and Russian
+// roulette uses a constant probability.  Replaying a path from its
+// counter-RNG seed therefore reproduces the same vertices, and every
+// contribution C_k of the path is a product that is linear in the
+// reflectance of each surface vertex before it (the BSDF weight fs/pdf, and
+// the NEE factor fsE at the vertex itself) and in the radiance of its
+// emitter.  With S = sum_k g_k C_k over the whole path (g_k = dL/dI at the
+// contribution's pixel times the film scale W*H/N) and P_j the value of that
+// sum when vertex j's reflectance first multiplies the path,
+//     dL/dR_m += sum_{j : prim_j = m} (S - P_j) / R_m      (per channel)
+//     dL/dLe_l += sum_{k : emitter_k = l} g_k C_k / Le_l.
+// Channels with R_m = 0 or Le_l = 0 get no gradient.
+//
+// Layout: the persistent engine's regenerating lanes; per-lane vertex
+// records (prim, P_j) in HBM as [record][lane] float4; per-block FP64 LDS
+// accumulators for the first NGI_ADJ_LDS_PRIMS primitives, flushed with one
+// FP64 global atomic per value and block; the rest go straight to global
+// FP64 atomics.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ngi_render.h"
+
+namespace ngi {
+
+#define NGI_ADJ_BLOCK 256
+#define NGI_ADJ_LDS_PRIMS 256
+
+struct AdjArgs {
+    const float* dLdI;     // W*H*3
+    float scale;           // W*H / N
+    float4* rec;           // [max_rec][lanes]
+    uint32_t lanes;
+    int max_rec;
+    double* gR;            // nprims*3
+    double* gLe;           // nprims*3
+    int nprims;
+};
+
+__device__ __forceinline__ void grad_add(double* lds, double* glob, int nlds, int prim, int c, float v) {
+    if (v == 0.0f) return;
+    if (prim < nlds) atomicAdd(lds + 3 * prim + c, (double)v);
+    else unsafeAtomicAdd(glob + 3 * prim + c, (double)v);
+}
+
+template <bool DIRECT, int WIDTH>
+__global__ void __launch_bounds__(NGI_ADJ_BLOCK) adjoint_kernel(RenderArgs a, AdjArgs g, unsigned long long* next) {
+    extern __shared__ double adj_smem[];
+    const int nlds = min(g.nprims, NGI_ADJ_LDS_PRIMS);
+    double* sR = adj_smem;
+    double* sL = adj_smem + 3 * nlds;
+    uint32_t* stk = reinterpret_cast<uint32_t*>(adj_smem + 6 * nlds);
+    for (int i = threadIdx.x; i < 6 * nlds; i += NGI_ADJ_BLOCK) adj_smem[i] = 0.0;
+    __syncthreads();
+
+    const DevScene& sc = a.sc;
+    const Cam<float> cam = load_cam<float>(sc.cam);
+    const DevPrim<float>* prims = prims_of<float>(sc);
+    const DevLight<float>* lights = lights_of<float>(sc);
+    const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
+    const uint32_t lane = __lane_id();
+    const uint32_t gid = blockIdx.x * NGI_ADJ_BLOCK + threadIdx.x;
+    uint32_t* my_stk = stk + threadIdx.x;
+
+    // Path state (as ProcessSample_PT/_PTDirect) plus the adjoint sums.
+    f3 T(0.0f), wi(0.0f), S(0.0f);
+    Surf<float> s;
+    int type = 0, prim = 0, nv = 0, pixel = 0, nrec = 0;
+    uint32_t c0 = 0, c1 = 0;
+    bool active = false, exhausted = false;
+
+    auto contribute = [&](int pix, f3 C, int emitter_prim) {
+        const float* gp = g.dLdI + 3 * (size_t)pix;
+        const f3 gC(gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale);
+        S = S + gC;
+        const DevPrim<float>& E = prims[emitter_prim];
+        if (E.Le[0] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 0, gC.x / E.Le[0]);
+        if (E.Le[1] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 1, gC.y / E.Le[1]);
+        if (E.Le[2] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 2, gC.z / E.Le[2]);
+    };
+    auto finish_path = [&]() {
+        for (int r = 0; r < nrec; ++r) {
+            const float4 e = g.rec[(size_t)r * g.lanes + gid];
+            const int m = __float_as_int(e.w);
+            const DevPrim<float>& P = prims[m];
+            const f3 d = S - f3(e.x, e.y, e.z);
+            if (P.R[0] != 0.0f) grad_add(sR, g.gR, nlds, m, 0, d.x / P.R[0]);
+            if (P.R[1] != 0.0f) grad_add(sR, g.gR, nlds, m, 1, d.y / P.R[1]);
+            if (P.R[2] != 0.0f) grad_add(sR, g.gR, nlds, m, 2, d.z / P.R[2]);
+        }
+    };
+
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= a.path_count) exhausted = true;
+                if (!active) {
+                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    if (local < a.path_count) {
+                        path_coords(a, a.path_begin + local, c0, c1, pixel);
+                        T = f3(1.0f);
+                        type = NGI_PRIM_E;
+                        prim = sc.sensor_prim;
+                        s.p = cam.eye;
+                        s.degenerate = true;
+                        wi = f3(0.0f);
+                        nv = 1;
+                        S = f3(0.0f);
+                        nrec = 0;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        // ---- one vertex (src/nanogi.cpp:483-606 / 645-801) ----
+        bool alive = nv < max_nv;
+        if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && nrec < g.max_rec) {
+            // Vertex reflectance enters here (NEE fsE, then the BSDF weight).
+            g.rec[(size_t)nrec * g.lanes + gid] = make_float4(S.x, S.y, S.z, __int_as_float(prim));
+            nrec++;
+        }
+        if (alive && DIRECT && sc.num_lights > 0) {
+            const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
+            TraceRay sr;
+            int target;
+            const f3 C = nee_contribution<float>(sc, cam, prims, type, prim, s, wi, T, rn, a.w, a.h, pixel, sr, target);
+            if (!is_zero(C)) {
+                TraceHit sh;
+                TraceCounters tc{0, 0};
+                if (!trace<WIDTH, true, false>(sc, sr, sh, tc, my_stk, NGI_ADJ_BLOCK)) {
+                    // The light picked by the NEE sample (same draw as sample_light).
+                    int li = (int)(u24f(rn.x) * (float)sc.num_lights);
+                    li = li < 0 ? 0 : (li > sc.num_lights - 1 ? sc.num_lights - 1 : li);
+                    contribute(target, C, lights[li].prim);
+                }
+            }
+        }
+        f3 wo(0.0f);
+        if (alive) {
+            const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
+            if (type == NGI_PRIM_E) {
+                wo = camera_sample<float>(a, cam, u24f(rx.x), u24f(rx.y), pixel);
+            } else {
+                f3 wgt;
+                alive = bsdf_sample<float>(prims[prim], type, s, wi, u24f(rx.x), u24f(rx.y), u24f(rx.z), wo, wgt);
+                if (alive) T = T * wgt;
+            }
+            if (alive) {
+                TraceRay er{s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, NGI_EPS_F, FLT_MAX};
+                TraceHit h;
+                TraceCounters tc{0, 0};
+                alive = trace<WIDTH, false, false>(sc, er, h, tc, my_stk, NGI_ADJ_BLOCK);
+                if (alive) {
+                    int hp;
+                    const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp);
+                    const DevPrim<float>& hm = prims[hp];
+                    if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA &&
+                        dot(ns.f.n, -wo) > 0.0f)
+                        contribute(pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                    if (a.rr_depth >= 0 && nv >= a.rr_depth) {
+                        if (u24f(rx.w) > a.rr_prob_f) alive = false;
+                        else T = T / a.rr_prob_f;
+                    }
+                    type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                    if (type == 0) alive = false;
+                    s = ns;
+                    prim = hp;
+                    wi = -wo;
+                    nv++;
+                }
+            }
+        }
+        if (!alive) {
+            finish_path();
+            active = false;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * nlds; i += NGI_ADJ_BLOCK) {
+        if (sR[i] != 0.0) unsafeAtomicAdd(g.gR + i, sR[i]);
+        if (sL[i] != 0.0) unsafeAtomicAdd(g.gLe + i, sL[i]);
+    }
+}
+
+template <bool DIRECT, int WIDTH>
+static int launch_adjoint(const RenderArgs& a, const AdjArgs& g, unsigned long long* next, unsigned grid, size_t lds, hipStream_t st) {
+    adjoint_kernel<DIRECT, WIDTH><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
+    NGI_HIP_CHECK(hipGetLastError());
+    return NGI_OK;
+}
+
+template <bool DIRECT, int WIDTH>
+static int adjoint_grid(size_t lds, unsigned& grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<DIRECT, WIDTH>, NGI_ADJ_BLOCK, lds));
+    grid = (unsigned)std::max(1, cus * std::max(1, per_cu));
+    return NGI_OK;
+}
+
+int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* dLdI, double* gR_out, double* gLe_out, ngi_stats* stats) {
+    Session S;
+    ngi_render_params q = p;
+    q.engine = NGI_ENGINE_PERSISTENT;
+    int st = S.init(scene, q);
+    if (st) {
+        S.destroy();
+        return st;
+    }
+    if (p.precision == NGI_PRECISION_FP64_ALL) {
+        S.destroy();
+        return fail(NGI_ERR_UNSUPPORTED, "the adjoint shades in FP32 (FP64 is used for the gradient reductions)");
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t str = S.stream();
+    const uint32_t npix = S.npix;
+    const uint64_t total = p.sampling == NGI_SAMPLING_PER_PIXEL ? (uint64_t)npix * (uint64_t)std::max(p.spp, 0) : (uint64_t)p.num_samples;
+    const int nprims = (int)scene->prims.size();
+    const bool direct = p.renderer == NGI_RENDERER_PTDIRECT;
+    const int width = scene->dev->view.bvh_width;
+    const int nlds = std::min(nprims, NGI_ADJ_LDS_PRIMS);
+    const size_t lds = (size_t)6 * nlds * 8 + (width == 8 ? (size_t)scene->dev->view.stack_entries * NGI_ADJ_BLOCK * 4 : 0);
+    unsigned grid = 1;
+    st = direct ? (width == 8 ? adjoint_grid<true, 8>(lds, grid) : adjoint_grid<true, 2>(lds, grid))
+                : (width == 8 ? adjoint_grid<false, 8>(lds, grid) : adjoint_grid<false, 2>(lds, grid));
+    std::vector<void*> mem;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(str);
+        for (void* m : mem) (void)hipFree(m);
+        S.destroy();
+    };
+    if (st) {
+        cleanup();
+        return st;
+    }
+    const int max_nv = (p.max_num_vertices < 0 || p.max_num_vertices > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : p.max_num_vertices;
+    AdjArgs g;
+    g.lanes = grid * NGI_ADJ_BLOCK;
+    g.max_rec = max_nv;
+    g.nprims = nprims;
+    g.scale = total ? (float)((double)npix / (double)total) : 0.0f;
+    std::vector<float> dl(3 * (size_t)npix);
+    for (size_t i = 0; i < dl.size(); ++i) dl[i] = (float)dLdI[i];
+    void *d_dl = nullptr, *d_rec = nullptr, *d_gR = nullptr, *d_gL = nullptr, *d_next = nullptr;
+#define NGI_ADJ_ALLOC(ptr, bytes)                                   \
+    do {                                                            \
+        hipError_t e_ = hipMalloc(&ptr, bytes);                     \
+        if (e_ != hipSuccess) {                                     \
+            cleanup();                                              \
+            return fail(NGI_ERR_OOM, "adjoint: hipMalloc failed");  \
+        }                                                           \
+        mem.push_back(ptr);                                         \
+    } while (0)
+    NGI_ADJ_ALLOC(d_dl, dl.size() * 4);
+    NGI_ADJ_ALLOC(d_rec, (size_t)g.lanes * g.max_rec * 16);
+    NGI_ADJ_ALLOC(d_gR, (size_t)nprims * 3 * 8 + 8);
+    NGI_ADJ_ALLOC(d_gL, (size_t)nprims * 3 * 8 + 8);
+    NGI_ADJ_ALLOC(d_next, 8);
+#undef NGI_ADJ_ALLOC
+    if (hipMemcpyAsync(d_dl, dl.data(), dl.size() * 4, hipMemcpyHostToDevice, str) != hipSuccess) {
+        cleanup();
+        return fail(NGI_ERR_DEVICE, "adjoint: upload failed");
+    }
+    st = S.zero(d_gR, (size_t)nprims * 3, true);
+    if (!st) st = S.zero(d_gL, (size_t)nprims * 3, true);
+    if (!st) st = S.zero(d_next, 1, true);
+    g.dLdI = reinterpret_cast<const float*>(d_dl);
+    g.rec = reinterpret_cast<float4*>(d_rec);
+    g.gR = reinterpret_cast<double*>(d_gR);
+    g.gLe = reinterpret_cast<double*>(d_gL);
+    RenderArgs a = S.args(nullptr);
+    a.path_begin = 0;
+    a.path_count = total;
+    a.sample_base = 0;
+    if (!st && total) {
+        unsigned long long* nx = reinterpret_cast<unsigned long long*>(d_next);
+        st = direct ? (width == 8 ? launch_adjoint<true, 8>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2>(a, g, nx, grid, lds, str))
+                    : (width == 8 ? launch_adjoint<false, 8>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2>(a, g, nx, grid, lds, str));
+    }
+    if (!st && hipMemcpyAsync(gR_out, d_gR, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");
+    if (!st && hipMemcpyAsync(gLe_out, d_gL, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");
+    if (!st && hipStreamSynchronize(str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: kernel failed");
+    if (!st) {
+        // Only parameters that exist: reflectance of D/G primitives, radiance of emitters.
+        for (int i = 0; i < nprims; ++i) {
+            const int ty = scene->prims[i].type;
+            if (!(ty & (NGI_PRIM_D | NGI_PRIM_G))) gR_out[3 * i] = gR_out[3 * i + 1] = gR_out[3 * i + 2] = 0.0;
+            if (!(ty & NGI_PRIM_L)) gLe_out[3 * i] = gLe_out[3 * i + 1] = gLe_out[3 * i + 2] = 0.0;
+        }
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->samples = (int64_t)total;
+            stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+    }
+    cleanup();
+    return st;
+}
+
+}  // namespace ngi
+
+using namespace ngi;
+
+extern "C" int ngi_render_adjoint(ngi_scene* scene, const ngi_render_params* p, const double* dL_dimage, double* grad_reflectance,
+                                  double* grad_Le, ngi_stats* stats) {
+    clear_error();
+    if (!scene || !p || !dL_dimage || !grad_reflectance || !grad_Le) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    return render_adjoint(scene, *p, dL_dimage, grad_reflectance, grad_Le, stats);
+}

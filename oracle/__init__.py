@@ -122,3 +122,18 @@ class LoadedScene:
         if getattr(self, "h", None):
             lib().oracle_scene_free(self.h)
             self.h = None
+
+
+def render_adjoint(scene, params, dLdI, num_threads=0):
+    """FP64 reference gradients (reflectance, Le), each (num_primitives, 3)."""
+    L = lib()
+    L.oracle_render_adjoint.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int]
+    L.oracle_render_adjoint.restype = ctypes.c_int
+    d = scene.raw_desc()
+    g = np.ascontiguousarray(dLdI, dtype=np.float64)
+    n = d.num_primitives
+    gR = np.zeros((n, 3))
+    gL = np.zeros((n, 3))
+    _check(L.oracle_render_adjoint(ctypes.addressof(d), ctypes.addressof(params), g.ctypes.data, gR.ctypes.data,
+                                   gL.ctypes.data, int(num_threads)))
+    return gR, gL

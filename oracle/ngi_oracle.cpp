@@ -615,8 +615,19 @@ struct Counts {
     long ext = 0, shadow = 0;
 };
 
+// Adjoint bookkeeping of one path (see oracle_render_adjoint).
+struct Adjoint {
+    const double* dLdI = nullptr;   // W*H*3
+    double scale = 1.0;             // W*H / N
+    std::vector<double>* gR = nullptr;
+    std::vector<double>* gLe = nullptr;
+    V S;
+    std::vector<std::pair<int, V>> rec;   // (primitive, S when its reflectance entered)
+};
+
 // One path sample: ProcessSample_PT (direct = false) or _PTDirect (true).
-void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt) {
+void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt,
+                    Adjoint* adj = nullptr) {
     const bool direct = prm.renderer == NGI_RENDERER_PTDIRECT;
     const int W = prm.width, H = prm.height;
     const uint32_t npix = (uint32_t)W * (uint32_t)H;
@@ -632,11 +643,26 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
         c1 = (uint32_t)(id >> 32) | 0x80000000u;
     }
     auto rng = [&](uint32_t vertex, uint32_t stream) { return philox(c0, c1, vertex, stream, prm.seed, prm.stream_id); };
-    auto add = [&](int pix, V c) {
-        film[3 * (size_t)pix] += c.x;
-        film[3 * (size_t)pix + 1] += c.y;
-        film[3 * (size_t)pix + 2] += c.z;
+    // Film write, or (adjoint) the gradient of dL/dI . I with respect to the
+    // emitter radiance and the running sum S of weighted contributions.
+    auto add = [&](int pix, V c, int emitter) {
+        if (!adj) {
+            film[3 * (size_t)pix] += c.x;
+            film[3 * (size_t)pix + 1] += c.y;
+            film[3 * (size_t)pix + 2] += c.z;
+            return;
+        }
+        const double* g = adj->dLdI + 3 * (size_t)pix;
+        const V gc(g[0] * c.x * adj->scale, g[1] * c.y * adj->scale, g[2] * c.z * adj->scale);
+        adj->S = adj->S + gc;
+        const double* Le = s.prims[emitter].p.Le;
+        for (int k = 0; k < 3; ++k)
+            if (Le[k] != 0) (*adj->gLe)[3 * (size_t)emitter + k] += gc[k] / Le[k];
     };
+    if (adj) {
+        adj->S = V();
+        adj->rec.clear();
+    }
     // Sensor (SampleEmitter(E): the last sensor, pdf 1) and its position.
     const Prim& E = s.prims[s.sensor];
     Geom geom;
@@ -651,6 +677,7 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
     const int maxv = (prm.max_num_vertices < 0 || prm.max_num_vertices > 1024) ? 1024 : prm.max_num_vertices;
     for (;;) {
         if (numVertices >= maxv) break;
+        if (adj && (type & (NGI_PRIM_D | NGI_PRIM_G))) adj->rec.emplace_back((int)(prim - &s.prims[0]), adj->S);
         if (direct && !s.lights.empty()) {
             const Q q = rng((uint32_t)numVertices, 1);
             const int n = (int)s.lights.size();
@@ -675,7 +702,7 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
                     RasterPosition(*prim, ppL, rx, ry);
                     index = PixelIndex(rx, ry, W, H);
                 }
-                add(index, Cu);
+                add(index, Cu, s.lights[li]);
             }
         }
         const Q q = rng((uint32_t)numVertices, 0);
@@ -703,7 +730,7 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
         const Prim& hp = s.prims[is.prim];
         if (!direct && (hp.p.type & NGI_PRIM_L)) {
             // EvaluateDirection(isect.geom, L, 0, -d, EL, false) * EvaluatePosition(isect.geom, false)
-            if (hp.p.light_type == NGI_LIGHT_AREA && LocalCos(is.geom.to_local(-wo)) > 0) add(pixelIndex, throughput * vec3(hp.p.Le));
+            if (hp.p.light_type == NGI_LIGHT_AREA && LocalCos(is.geom.to_local(-wo)) > 0) add(pixelIndex, throughput * vec3(hp.p.Le), is.prim);
         }
         if (prm.rr_depth >= 0 && numVertices >= prm.rr_depth) {
             if (urr > prm.rr_prob) break;
@@ -714,6 +741,14 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
         type = hp.p.type & ~(NGI_PRIM_L | NGI_PRIM_E);
         wi = -wo;
         numVertices++;
+    }
+    if (adj) {
+        for (const auto& r : adj->rec) {
+            const ngi_primitive& P = s.prims[r.first].p;
+            const double* R = (P.type & NGI_PRIM_D) ? P.D_R : P.G_R;
+            for (int k = 0; k < 3; ++k)
+                if (R[k] != 0) (*adj->gR)[3 * (size_t)r.first + k] += (adj->S[k] - r.second[k]) / R[k];
+        }
     }
 }
 
@@ -862,6 +897,52 @@ int oracle_scene_render_paths(void* h, const ngi_render_params* prm, uint64_t pa
     for (int k = 0; k < T; ++k) {
         for (size_t i = 0; i < np; ++i) film[i] += films[k][i];
         if (counts) counts[0] += cnts[k].ext, counts[1] += cnts[k].shadow;
+    }
+    return NGI_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// FP64 reference gradient of L = sum_p dLdI_p . I_p with respect to each
+// primitive's reflectance (D.R or G.R) and radiance (Le), by replaying the
+// same paths as oracle_render (see ngi_adjoint.hip for the estimator).
+int oracle_render_adjoint(const ngi_scene_desc* d, const ngi_render_params* prm, const double* dLdI, double* gR, double* gLe, int num_threads) {
+    if (!d || !prm || !dLdI || !gR || !gLe) return NGI_ERR_INVALID_ARG;
+    Scene s;
+    int st = load(s, d, true);
+    if (st) return st;
+    if (s.sensor < 0) return NGI_ERR_UNSUPPORTED;
+    const uint64_t npix = (uint64_t)prm->width * prm->height;
+    const uint64_t total = prm->sampling == NGI_SAMPLING_PER_PIXEL ? npix * (uint64_t)prm->spp : (uint64_t)prm->num_samples;
+    const size_t np = d->num_primitives;
+    int T = num_threads > 0 ? num_threads : (int)std::thread::hardware_concurrency();
+    T = std::max(1, T);
+    std::vector<std::vector<double>> R(T, std::vector<double>(3 * np, 0.0)), L(T, std::vector<double>(3 * np, 0.0));
+    std::atomic<uint64_t> next(0);
+    auto work = [&](int k) {
+        Adjoint adj;
+        adj.dLdI = dLdI;
+        adj.scale = (double)npix / (double)total;
+        adj.gR = &R[k];
+        adj.gLe = &L[k];
+        std::vector<double> dummy;
+        Counts cnt;
+        for (;;) {
+            const uint64_t b = next.fetch_add(256);
+            if (b >= total) break;
+            for (uint64_t i = b; i < std::min(total, b + 256); ++i) process_sample(s, *prm, i, dummy, cnt, &adj);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < T; ++k) th.emplace_back(work, k);
+    work(0);
+    for (auto& t : th) t.join();
+    for (size_t i = 0; i < 3 * np; ++i) {
+        gR[i] = 0;
+        gLe[i] = 0;
+        for (int k = 0; k < T; ++k) gR[i] += R[k][i], gLe[i] += L[k][i];
     }
     return NGI_OK;
 }
