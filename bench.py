@@ -44,6 +44,194 @@ def measured_traffic(a, engine, precision):
     return None
 
 
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def bench_batch(a):
+    """Many small scenes in one persistent launch against one launch per scene.
+
+    256 seeded Cornell-style rooms with boxes and icospheres (10-5,000
+    triangles, log-uniform), square images of 64-512 pixels a side and
+    16-512 spp (log-uniform), ptdirect, at most 16 segments, Russian roulette
+    from depth 4.  Completion time of a scene = time from the start of the
+    frame until its last path ended (device clock for the batch; host clock
+    after each launch's synchronisation for the per-scene baseline)."""
+    import numpy as np
+
+    import nanogi_amd as ngi
+    rng = np.random.default_rng(2024)
+    lu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    scenes, params = [], []
+    t0 = time.perf_counter()
+    for i in range(a.batch_scenes):
+        tris = int(round(lu(10, 5000)))
+        side = int(round(lu(64, 512)))
+        spp = int(round(lu(16, 512)))
+        s = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 1000 + i, 1.0, max(tris, 12))
+        s.build(8)
+        scenes.append(s)
+        p = ngi.default_params(renderer="ptdirect", width=side, height=side, sampling=ngi.SAMPLING_PER_PIXEL, spp=spp,
+                               max_num_vertices=17, rr_depth=4, rr_prob=0.5, seed=77, engine=ngi.ENGINE_PERSISTENT)
+        p.stream_id = i
+        params.append(p)
+    gen_s = time.perf_counter() - t0
+    total = sum(p.width * p.height * p.spp for p in params)
+    tris = [sc.num_triangles for sc in scenes]
+    log(f"batch: {len(scenes)} scenes, {total} paths, {min(tris)}-{max(tris)} triangles, generated in {gen_s:.2f} s")
+
+    t0 = time.perf_counter()
+    b = ngi.Batch(scenes, params)
+    create_s = time.perf_counter() - t0
+    cfg = b.config()
+    log(f"batch: autotuned block {cfg['block_size']}, {cfg['blocks_per_cu']} blocks/CU in {cfg['tune_ms'] / 1e3:.2f} s")
+    b.render(0)   # warm-up frame
+    b.sync()
+    frames = 2
+    t0 = time.perf_counter()
+    for f in range(1, frames + 1):
+        b.render(f)
+    b.sync()
+    batch_s = (time.perf_counter() - t0) / frames
+    ms, st = b.times()   # completion times of the last frame
+    batch_rate = total / batch_s
+    log(f"batch: {batch_rate / 1e6:.1f} M samples/s, frame {batch_s * 1e3:.1f} ms")
+
+    # Baseline: one persistent launch per scene, in the same order.
+    sessions = [ngi.Session(sc, p) for sc, p in zip(scenes, params)]
+    for se, p in zip(sessions[:4], params[:4]):
+        se.render(p.spp)
+        se.sync()
+    for se in sessions:
+        se.reset()
+    done = []
+    t0 = time.perf_counter()
+    for se, p in zip(sessions, params):
+        se.render(p.spp)
+        se.sync()
+        done.append((time.perf_counter() - t0) * 1e3)
+    single_s = done[-1] / 1e3
+    for se in sessions:
+        se.close()
+    b.close()
+    log(f"one launch per scene: {total / single_s / 1e6:.1f} M samples/s, {single_s * 1e3:.1f} ms")
+    return {
+        "workload": f"{len(scenes)} Cornell-style rooms, {min(tris)}-{max(tris)} triangles, 64-512 px square, 16-512 spp "
+                    f"(log-uniform, seed 2024), ptdirect, max 16 segments, RR from depth 4",
+        "paths": total, "samples_per_s": round(batch_rate, 1), "frame_ms": round(batch_s * 1e3, 3),
+        "rays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / batch_s, 1),
+        "completion_ms": {"p50": round(pct(ms, 50), 3), "p99": round(pct(ms, 99), 3), "max": round(float(np.max(ms)), 3)},
+        "one_launch_per_scene": {"samples_per_s": round(total / single_s, 1), "total_ms": round(single_s * 1e3, 3),
+                                 "completion_ms": {"p50": round(pct(done, 50), 3), "p99": round(pct(done, 99), 3)}},
+        "speedup": round(single_s / batch_s, 3),
+        "launch": {"graph": bool(cfg["use_graph"]), "block_size": cfg["block_size"], "blocks_per_cu": cfg["blocks_per_cu"],
+                   "autotune_ms": round(cfg["tune_ms"], 1), "create_ms": round(create_s * 1e3, 1)},
+    }
+
+
+def bench_adjoint(a):
+    """Inverse rendering of the ~77k-triangle nave: 6 albedos x 3 + 4 light
+    radiances x 3 from a 512x512 target (ptdirect, 64 spp forward + 64 spp
+    replayed adjoint with a decorrelated seed, L2 loss, Adam)."""
+    import numpy as np
+
+    import nanogi_amd as ngi
+    W = H = 512
+    scene = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
+    scene.build(8)
+    pr = scene.desc()["primitives"]
+    is_r = (pr["type"] & ngi.PRIM_D) > 0
+    is_l = (pr["type"] & ngi.PRIM_L) > 0
+    R_true = np.where(is_r[:, None], pr["D_R"], 0.0)
+    L_true = pr["Le"].copy()
+
+    def params(seed, spp=64):
+        return ngi.default_params(renderer="ptdirect", width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=spp,
+                                  max_num_vertices=17, rr_depth=4, rr_prob=0.5, seed=seed, engine=ngi.ENGINE_PERSISTENT)
+
+    target, _ = ngi.render(scene, params(999, 256))
+    R = np.where(is_r[:, None], 0.5, 0.0)
+    L = np.where(is_l[:, None], 10.0, 0.0)
+
+    def loss_grad(R, L, seed, adj_seed):
+        scene.set_params(R, L)
+        img, _ = ngi.render(scene, params(seed))
+        d = img - target
+        g = 2.0 * d / d.size
+        gR, gL, _ = ngi.render_adjoint(scene, params(adj_seed), g)
+        return float(np.mean(d * d)), gR, gL
+
+    # Gradient check at the start point: adjoint on the forward's own paths
+    # against central differences with common random numbers (h = 1e-3).
+    scene.set_params(R, L)
+    img0, _ = ngi.render(scene, params(5))
+    g0 = 2.0 * (img0 - target) / img0.size
+    gR0, gL0, _ = ngi.render_adjoint(scene, params(5), g0)
+    fd_err = []
+    checks = [("R", int(np.nonzero(is_r)[0][k]), c) for k, c in ((0, 0), (2, 1), (4, 2))] + \
+             [("L", int(np.nonzero(is_l)[0][k]), c) for k, c in ((0, 0), (3, 2))]
+    for which, m, c in checks:
+        vals = []
+        for sgn in (1, -1):
+            R2, L2 = R.copy(), L.copy()
+            (R2 if which == "R" else L2)[m, c] += sgn * 1e-3
+            scene.set_params(R2, L2)
+            vals.append(float(np.sum(g0 * ngi.render(scene, params(5))[0])))
+        fd = (vals[0] - vals[1]) / 2e-3
+        an = (gR0 if which == "R" else gL0)[m, c]
+        fd_err.append(abs(an - fd) / max(abs(fd), 1e-12))
+
+    # Cost of one forward and one adjoint pass (synchronous, same workload).
+    scene.set_params(R, L)
+    reps = 5
+    t0 = time.perf_counter()
+    for k in range(reps):
+        ngi.render(scene, params(100 + k))
+    fwd_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for k in range(reps):
+        ngi.render_adjoint(scene, params(200 + k), g0)
+    adj_s = (time.perf_counter() - t0) / reps
+
+    # Adam over albedos and radiances.
+    mR, vR, mL, vL = (np.zeros_like(R) for _ in range(4))
+    lrR, lrL, b1, b2, eps = 0.01, 0.2, 0.9, 0.999, 1e-8
+    losses = []
+    t0 = time.perf_counter()
+    for it in range(a.adjoint_steps):
+        loss, gR, gL = loss_grad(R, L, 10_000 + 2 * it, 10_001 + 2 * it)
+        losses.append(loss)
+        gR = np.where(is_r[:, None], gR, 0.0)
+        gL = np.where(is_l[:, None], gL, 0.0)
+        for x, g, m, v, lr in ((R, gR, mR, vR, lrR), (L, gL, mL, vL, lrL)):
+            m *= b1
+            m += (1 - b1) * g
+            v *= b2
+            v += (1 - b2) * g * g
+            x -= lr * (m / (1 - b1 ** (it + 1))) / (np.sqrt(v / (1 - b2 ** (it + 1))) + eps)
+        np.clip(R, 0.0, 1.0, out=R)
+        np.clip(L, 0.0, None, out=L)
+        if (it + 1) % 50 == 0:
+            log(f"adjoint: Adam step {it + 1}/{a.adjoint_steps}, loss {loss:.4g}")
+    opt_s = time.perf_counter() - t0
+    scene.set_params(R_true, L_true)
+    rel = lambda x, y, m: float(np.linalg.norm((x - y)[m]) / np.linalg.norm(y[m]))
+    return {
+        "workload": f"nave ({scene.num_triangles} triangles), 512x512, ptdirect 64 spp forward + 64 spp adjoint "
+                    f"(decorrelated seeds), max 16 segments, RR from depth 4; 6x3 albedo + 4x3 radiance parameters",
+        "evals_per_s": round(a.adjoint_steps / opt_s, 3) if a.adjoint_steps else None,
+        "forward_ms": round(fwd_s * 1e3, 3), "adjoint_ms": round(adj_s * 1e3, 3),
+        "adjoint_over_forward": round(adj_s / fwd_s, 3),
+        "fd_rel_err_max": float(max(fd_err)), "fd_checks": len(fd_err),
+        "adam_steps": a.adjoint_steps,
+        "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+        "param_rel_err": {"albedo": rel(R, R_true, is_r), "radiance": rel(L, L_true, is_l)},
+        "param_rel_err_start": {"albedo": rel(np.where(is_r[:, None], 0.5, 0.0), R_true, is_r),
+                                "radiance": rel(np.where(is_l[:, None], 10.0, 0.0), L_true, is_l)},
+    }
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +250,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-counters", action="store_true")
     ap.add_argument("--paths-in-flight", type=int, default=0, help="wavefront queue capacity (0 = auto)")
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="scenes of the batched many-scene workload reported beside the main line (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="Adam steps of the inverse-rendering workload reported beside the main line (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -241,6 +433,13 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
 
+    batch = None
+    if a.batch_scenes > 0:
+        batch = bench_batch(a)
+    adjoint = None
+    if a.adjoint_steps > 0:
+        adjoint = bench_adjoint(a)
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(elapsed * 1e3 / a.steps, 3), "higher_is_better": True,
@@ -259,6 +458,8 @@ def main():
         "cpu_baseline": cpu,
         "counters": counts,
         "bvh_build_s": round(info["build_ms"] / 1e3, 3),
+        "batch": batch,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     return 0

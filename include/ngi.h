@@ -298,9 +298,43 @@ int ngi_render_adjoint(ngi_scene* scene, const ngi_render_params* p, const doubl
 /* Batched rendering of many independent scenes in one persistent      */
 /* launch (no reference counterpart; the reference runs one scene per  */
 /* process).  films[i] receives scene i's W_i*H_i*3 normalised film.   */
+/* Every scene renders pt, or every scene ptdirect, in FP32_FP64ACC    */
+/* with BVHs of one width; other engines and precisions are rejected.  */
+/* Scene i's paths are the ones ngi_render draws for params[i] (same   */
+/* seed and stream id give the same image).  scene_ms (optional,       */
+/* num_scenes doubles) receives each scene's completion time in ms     */
+/* after the launch started, from the device wall clock.               */
 /* ------------------------------------------------------------------ */
 int ngi_render_batch(ngi_scene* const* scenes, const ngi_render_params* params, int num_scenes,
                      double* const* films, double* scene_ms, ngi_stats* stats);
+
+/* Reusable batch (no reference counterpart): device films and per-scene   */
+/* arguments stay allocated, and each ngi_batch_render replays one         */
+/* captured hipGraph (film/counter fill kernel + persistent launch).       */
+/* Scenes must outlive the batch; a scene rebuilt with a deeper BVH or     */
+/* another BVH width than at creation makes ngi_batch_render fail.         */
+typedef struct ngi_batch_opts {
+    int block_size;      /* 64, 128 or 256 threads; 0 = autotune */
+    int blocks_per_cu;   /* resident blocks per CU (clamped to occupancy); 0 = autotune */
+    int use_graph;       /* 1 = replay a captured hipGraph, 0 = plain launches */
+    int64_t tune_paths;  /* paths per autotuner trial, drawn proportionally from every scene */
+} ngi_batch_opts;
+typedef struct ngi_batch ngi_batch;
+void ngi_batch_opts_default(ngi_batch_opts* o);   /* autotune both, graph on, 2^24 paths per trial */
+int ngi_batch_create(ngi_scene* const* scenes, const ngi_render_params* params, int num_scenes,
+                     const ngi_batch_opts* opts, ngi_batch** out);
+/* Enqueue one frame (asynchronous).  Scene i uses seed params[i].seed +  */
+/* frame, so frame 0 reproduces ngi_render for params[i].                  */
+int ngi_batch_render(ngi_batch* b, uint32_t frame);
+int ngi_batch_sync(ngi_batch* b);
+/* Normalised film of scene `scene` from the last frame (W*H*3 doubles). */
+int ngi_batch_get_film(ngi_batch* b, int scene, double* film_out);
+/* Per-scene completion times (ms after the launch started, device wall  */
+/* clock) and totals of the last frame; either output may be NULL.       */
+int ngi_batch_get_times(ngi_batch* b, double* scene_ms, ngi_stats* stats);
+/* Launch configuration in use (after autotuning) and the tuning time.    */
+int ngi_batch_get_opts(ngi_batch* b, ngi_batch_opts* chosen, double* tune_ms);
+void ngi_batch_destroy(ngi_batch* b);
 
 #ifdef __cplusplus
 }

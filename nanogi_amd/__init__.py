@@ -9,6 +9,7 @@ two path tracers: ``Scene.load_yaml`` is ``Scene::Load``
 (rt.hpp:2162-2261).  Every call runs the native library; when it is missing
 the import fails loudly -- there is no Python or CPU fallback.
 """
+import copy
 import ctypes
 import os
 
@@ -100,6 +101,10 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("block_size", c_int), ("blocks_per_cu", c_int), ("use_graph", c_int), ("tune_paths", c_int64)]
+
+
 # Every entry point declared in include/ngi.h (checked by the CPU tests).
 EXPORTS = [
     "ngi_last_error", "ngi_device_count", "ngi_scene_load_yaml", "ngi_scene_create", "ngi_scene_generate",
@@ -108,6 +113,8 @@ EXPORTS = [
     "ngi_render_params_default", "ngi_render", "ngi_session_create", "ngi_session_render",
     "ngi_session_render_batch", "ngi_session_sync", "ngi_session_get_film", "ngi_session_get_stats",
     "ngi_session_reset", "ngi_session_destroy", "ngi_render_adjoint", "ngi_render_batch", "ngi_image_save",
+    "ngi_batch_opts_default", "ngi_batch_create", "ngi_batch_render", "ngi_batch_sync", "ngi_batch_get_film",
+    "ngi_batch_get_times", "ngi_batch_get_opts", "ngi_batch_destroy",
 ]
 
 
@@ -160,6 +167,14 @@ def lib():
         "ngi_render_adjoint": ([vp, P(RenderParams), vp, vp, vp, P(Stats)], c_int),
         "ngi_render_batch": ([vp, P(RenderParams), c_int, vp, vp, P(Stats)], c_int),
         "ngi_image_save": ([ctypes.c_char_p, vp, c_int, c_int], c_int),
+        "ngi_batch_opts_default": ([P(BatchOpts)], None),
+        "ngi_batch_create": ([vp, P(RenderParams), c_int, P(BatchOpts), P(vp)], c_int),
+        "ngi_batch_render": ([vp, c_uint32], c_int),
+        "ngi_batch_sync": ([vp], c_int),
+        "ngi_batch_get_film": ([vp, c_int, vp], c_int),
+        "ngi_batch_get_times": ([vp, vp, P(Stats)], c_int),
+        "ngi_batch_get_opts": ([vp, P(BatchOpts), P(c_double)], c_int),
+        "ngi_batch_destroy": ([vp], None),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -395,6 +410,88 @@ def render(scene, params=None, **kw):
     st = Stats()
     _check(lib().ngi_render(scene.handle, ctypes.byref(p), _ptr(film), ctypes.byref(st)))
     return film[:p.width * p.height * 3].reshape(p.height, p.width, 3), st.as_dict()
+
+
+def render_batch(scenes, params):
+    """Render many scenes in one persistent launch (ngi_render_batch).
+
+    `params` is one RenderParams per scene.  Returns (films, scene_ms, stats):
+    films[i] is scene i's film[H_i, W_i, 3] (as render() returns it) and
+    scene_ms[i] its completion time in ms after the launch started."""
+    n = len(scenes)
+    if n != len(params):
+        raise ValueError("one RenderParams per scene")
+    films = [np.zeros(max(p.width * p.height * 3, 1), dtype=np.float64) for p in params]
+    ptrs = (ctypes.c_void_p * max(n, 1))(*[f.ctypes.data for f in films])
+    handles = (ctypes.c_void_p * max(n, 1))(*[s.handle for s in scenes])
+    parr = (RenderParams * max(n, 1))(*params)
+    ms = np.zeros(max(n, 1))
+    st = Stats()
+    _check(lib().ngi_render_batch(handles, parr, n, ptrs, _ptr(ms), ctypes.byref(st)))
+    out = [f[:p.width * p.height * 3].reshape(p.height, p.width, 3) for f, p in zip(films, params)]
+    return out, ms[:n], st.as_dict()
+
+
+def batch_opts(**kw):
+    """BatchOpts with the library defaults (autotuned launch, graph replay), then `kw`."""
+    o = BatchOpts()
+    lib().ngi_batch_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    """Reusable many-scene render (ngi_batch_*): one graph replay per frame.
+
+    Holds references to its scenes so they outlive the device batch."""
+
+    def __init__(self, scenes, params, opts=None):
+        if len(scenes) != len(params) or not scenes:
+            raise ValueError("one RenderParams per scene, at least one scene")
+        self.scenes = list(scenes)
+        self.params = [copy.copy(p) for p in params]
+        n = len(scenes)
+        handles = (ctypes.c_void_p * n)(*[s.handle for s in scenes])
+        parr = (RenderParams * n)(*self.params)
+        h = ctypes.c_void_p()
+        _check(lib().ngi_batch_create(handles, parr, n, ctypes.byref(opts) if opts is not None else None, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.ngi_batch_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def render(self, frame=0):
+        _check(lib().ngi_batch_render(self._h, int(frame)))
+
+    def sync(self):
+        _check(lib().ngi_batch_sync(self._h))
+
+    def film(self, i):
+        p = self.params[i]
+        f = np.zeros(p.width * p.height * 3)
+        _check(lib().ngi_batch_get_film(self._h, int(i), _ptr(f)))
+        return f.reshape(p.height, p.width, 3)
+
+    def times(self):
+        """(per-scene completion ms of the last frame, stats dict)."""
+        ms = np.zeros(len(self.scenes))
+        st = Stats()
+        _check(lib().ngi_batch_get_times(self._h, _ptr(ms), ctypes.byref(st)))
+        return ms, st.as_dict()
+
+    def config(self):
+        o = BatchOpts()
+        t = c_double()
+        _check(lib().ngi_batch_get_opts(self._h, ctypes.byref(o), ctypes.byref(t)))
+        return {"block_size": o.block_size, "blocks_per_cu": o.blocks_per_cu, "use_graph": o.use_graph,
+                "tune_paths": o.tune_paths, "tune_ms": t.value}
 
 
 def save_image(path, film):

@@ -1,0 +1,563 @@
+// Batched rendering of many independent scenes in one persistent launch
+// (no reference counterpart: nanogi renders one scene per process).
+//
+// Every scene's paths are numbered consecutively in one global path space
+// (prefix offsets per scene).  Lanes of the persistent kernel take the next
+// global path id as soon as their path ends, find its scene with a binary
+// search over the offsets, and run the same vertex loop as the single-scene
+// engines (path_vertex) on that scene's BVH, film and parameters.  Small and
+// large scenes therefore share the GPU instead of leaving it idle between
+// per-scene launches.  The lane that completes a scene's last path records a
+// device wall-clock time stamp, which gives per-scene completion times.
+//
+// A batch owns its device memory (all films in one buffer, per-scene
+// arguments, counters) and replays one captured hipGraph per frame: a fill
+// kernel that clears films and counters, then the persistent kernel.  The
+// block size and the number of resident blocks per CU are picked by timing
+// candidates on a proportional sample of every scene's paths.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ngi_render.h"
+
+namespace ngi {
+
+struct BatchDev {
+    const RenderArgs* args;           // per scene
+    const unsigned long long* offset; // per scene + 1: first global path id
+    unsigned long long* done;         // per scene: completed paths
+    unsigned long long* t_done;       // per scene: wall clock at completion
+    unsigned long long* misc;         // MISC_* words
+    unsigned long long total;
+    int num_scenes;
+};
+
+// misc words: wall clock at start (atomic min), global path counter, ray counters.
+enum { MISC_T_START = 0, MISC_NEXT = 1, MISC_EXT = 2, MISC_SHADOW = 3, MISC_N = 4 };
+
+__device__ __forceinline__ void wave_sum_add(unsigned long long* p, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(p, x);
+}
+
+template <bool DIRECT, int WIDTH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) batch_kernel(BatchDev b) {
+    extern __shared__ uint32_t stk[];   // max stack_entries over the scenes x BLOCK (wide BVH only)
+    if (threadIdx.x == 0) atomicMin(b.misc + MISC_T_START, (unsigned long long)wall_clock64());
+    const uint32_t lane = __lane_id();
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    PathState<float> ps;
+    int si = 0;
+    bool active = false, exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(b.misc + MISC_NEXT, (unsigned long long)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= b.total) exhausted = true;
+                if (!active) {
+                    const unsigned long long gid = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    if (gid < b.total) {
+                        // Scene of this path: the last offset <= gid (scenes
+                        // with no paths share their offset with the next one).
+                        int lo = 0, hi = b.num_scenes;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (b.offset[mid] <= gid) lo = mid;
+                            else hi = mid;
+                        }
+                        si = lo;
+                        const RenderArgs& a = b.args[si];
+                        path_start<float>(a, gid - b.offset[si], load_cam<float>(a.sc.cam), ps);
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+        const RenderArgs& a = b.args[si];
+        active = path_vertex<float, double, DIRECT, WIDTH, false>(a, load_cam<float>(a.sc.cam), prims_of<float>(a.sc), ps, stk + threadIdx.x,
+                                                                  BLOCK, clamp_vertices(a.max_nv), cnt);
+        if (!active) {
+            // The lane that ends a scene's last path stamps its completion time.
+            const unsigned long long n = b.offset[si + 1] - b.offset[si];
+            if (atomicAdd(b.done + si, 1ull) + 1ull == n) b.t_done[si] = (unsigned long long)wall_clock64();
+        }
+    }
+    wave_sum_add(b.misc + MISC_EXT, cnt[CNT_EXT_RAYS]);
+    wave_sum_add(b.misc + MISC_SHADOW, cnt[CNT_SHADOW_RAYS]);
+}
+
+// Clears the films and per-scene counters and resets the path counter: the
+// first node of the captured graph (a kernel, not a memset node).
+__global__ void batch_clear(double* film, size_t film_n, unsigned long long* done, unsigned long long* t_done, int n, unsigned long long* misc) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = t; i < film_n; i += stride) film[i] = 0.0;
+    for (size_t i = t; i < (size_t)n; i += stride) done[i] = 0ull, t_done[i] = 0ull;
+    if (t < MISC_N) misc[t] = t == MISC_T_START ? ~0ull : 0ull;
+}
+
+using BatchKernel = void (*)(BatchDev);
+
+template <bool D, int W>
+static BatchKernel pick_block(int block) {
+    switch (block) {
+        case 64: return batch_kernel<D, W, 64>;
+        case 128: return batch_kernel<D, W, 128>;
+        case 256: return batch_kernel<D, W, 256>;
+        default: return nullptr;
+    }
+}
+
+static BatchKernel pick_kernel(bool direct, int width, int block) {
+    if (direct) return width == 8 ? pick_block<true, 8>(block) : pick_block<true, 2>(block);
+    return width == 8 ? pick_block<false, 8>(block) : pick_block<false, 2>(block);
+}
+
+struct Batch {
+    std::vector<ngi_scene*> scenes;
+    std::vector<ngi_render_params> params;
+    std::vector<RenderArgs> args;
+    std::vector<unsigned long long> offset;     // n + 1
+    std::vector<size_t> film_off;               // n + 1, in doubles
+    std::vector<const void*> nodes_seen;        // device BVH of each scene at the last upload
+    std::vector<uint64_t> version_seen;
+    ngi_batch_opts opts;
+    bool direct = false;
+    int width = 0;
+    uint32_t stack = 0;
+    int num_cus = 0, rate_khz = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* film = nullptr;
+    RenderArgs* d_args = nullptr;
+    unsigned long long *d_off = nullptr, *d_done = nullptr, *d_tdone = nullptr, *d_misc = nullptr, *d_tune_off = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t frame = 0;
+    bool rendered = false;
+    double tune_ms = 0.0;
+
+    int n() const { return (int)scenes.size(); }
+    BatchDev dev(const unsigned long long* off, unsigned long long total) const {
+        BatchDev b;
+        b.args = d_args;
+        b.offset = off;
+        b.done = d_done;
+        b.t_done = d_tdone;
+        b.misc = d_misc;
+        b.total = total;
+        b.num_scenes = n();
+        return b;
+    }
+    size_t lds(int block) const { return width == 8 ? (size_t)stack * block * 4 : 0; }
+    int max_per_cu(int block) const {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_kernel(direct, width, block), block, lds(block)) != hipSuccess) return 0;
+        return occ;
+    }
+    int enqueue_clear() const {
+        const size_t fn = film_off.back();
+        const int blocks = (int)std::min<size_t>(4096, std::max<size_t>(1, (fn + 255) / 256));
+        batch_clear<<<blocks, 256, 0, stream>>>(film, fn, d_done, d_tdone, n(), d_misc);
+        NGI_HIP_CHECK(hipGetLastError());
+        return NGI_OK;
+    }
+    int enqueue_render(const unsigned long long* off, unsigned long long total, int block, int per_cu) const {
+        if (!total) return NGI_OK;
+        BatchKernel k = pick_kernel(direct, width, block);
+        if (!k) return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", block);
+        const unsigned grid = (unsigned)std::max(1, num_cus * per_cu);
+        k<<<grid, block, lds(block), stream>>>(dev(off, total));
+        NGI_HIP_CHECK(hipGetLastError());
+        return NGI_OK;
+    }
+    int refresh();
+    int autotune();
+    int capture();
+    int create(ngi_scene* const* sc, const ngi_render_params* p, int count, const ngi_batch_opts& o);
+    int render(uint32_t frame);
+    int sync();
+    int get_film(int i, double* out);
+    int get_times(double* scene_ms, ngi_stats* st);
+    void destroy();
+};
+
+static int validate(ngi_scene* const* scenes, const ngi_render_params* params, int n) {
+    if (n <= 0) return fail(NGI_ERR_INVALID_ARG, "no scenes");
+    const bool direct = params[0].renderer == NGI_RENDERER_PTDIRECT;
+    for (int i = 0; i < n; ++i) {
+        const ngi_render_params& p = params[i];
+        const ngi_scene* sc = scenes[i];
+        if (!sc) return fail(NGI_ERR_INVALID_ARG, "scene %d is null", i);
+        if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT)
+            return fail(NGI_ERR_UNSUPPORTED, "scene %d: batches render pt or ptdirect only", i);
+        if ((p.renderer == NGI_RENDERER_PTDIRECT) != direct)
+            return fail(NGI_ERR_UNSUPPORTED, "scene %d: every scene of a batch uses the same renderer", i);
+        if (p.precision != NGI_PRECISION_FP32_FP64ACC)
+            return fail(NGI_ERR_UNSUPPORTED, "scene %d: batches render with FP32 shading and an FP64 film", i);
+        if (p.sampling != NGI_SAMPLING_PER_PIXEL && p.sampling != NGI_SAMPLING_REFERENCE_UNIFORM)
+            return fail(NGI_ERR_INVALID_ARG, "scene %d: unknown sampling mode %d", i, p.sampling);
+        if (p.width <= 0 || p.height <= 0 || (int64_t)p.width * p.height > (1ll << 28))
+            return fail(NGI_ERR_INVALID_ARG, "scene %d: bad image size", i);
+        if (p.sampling == NGI_SAMPLING_PER_PIXEL ? p.spp <= 0 : p.num_samples <= 0)
+            return fail(NGI_ERR_INVALID_ARG, "scene %d: sample count must be positive", i);
+        if (p.max_num_vertices < -1) return fail(NGI_ERR_INVALID_ARG, "scene %d: bad max_num_vertices", i);
+        if (p.rr_depth >= 0 && !(p.rr_prob > 0.0 && p.rr_prob <= 1.0))
+            return fail(NGI_ERR_INVALID_ARG, "scene %d: Russian roulette probability must be in (0, 1]", i);
+        if (sc->sensor_prim < 0) return fail(NGI_ERR_INVALID_SCENE, "scene %d has no sensor (E) primitive", i);
+        if (sc->prims[sc->sensor_prim].sensor_type != NGI_SENSOR_PINHOLE)
+            return fail(NGI_ERR_UNSUPPORTED, "scene %d: only pinhole sensors are supported", i);
+    }
+    return NGI_OK;
+}
+
+// (Re)uploads every scene and the per-scene arguments.  Scenes whose BVH or
+// parameters changed since the last upload are re-read; the traversal stack
+// of the captured launch must still cover every scene's BVH depth.
+int Batch::refresh() {
+    bool changed = false;
+    uint32_t need_stack = 0;
+    int w = 0;
+    for (int i = 0; i < n(); ++i) {
+        ngi_scene* sc = scenes[i];
+        int st = device_prepare(*sc);
+        if (st) return st;
+        const DevScene& v = sc->dev->view;
+        if (w && v.bvh_width != w) return fail(NGI_ERR_INVALID_ARG, "all scenes of a batch need the same BVH width");
+        w = v.bvh_width;
+        need_stack = std::max(need_stack, v.stack_entries);
+        if (nodes_seen[i] != (const void*)v.nodes || version_seen[i] != sc->params_version) {
+            NGI_HIP_CHECK(hipStreamSynchronize(sc->dev->stream));   // uploads run on the scene's stream
+            nodes_seen[i] = v.nodes;
+            version_seen[i] = sc->params_version;
+            changed = true;
+        }
+    }
+    if (d_args && (w != width || need_stack > stack))
+        return fail(NGI_ERR_INVALID_ARG, "a scene was rebuilt with another BVH width or a deeper BVH than the batch was created for");
+    width = w;
+    stack = std::max(stack, need_stack);
+    for (int i = 0; i < n(); ++i) {
+        RenderArgs& a = args[i];
+        a.sc = scenes[i]->dev->view;
+        const uint32_t seed = params[i].seed + frame;
+        if (a.seed != seed) changed = true;
+        a.seed = seed;
+    }
+    if (changed && d_args) NGI_HIP_CHECK(hipMemcpyAsync(d_args, args.data(), args.size() * sizeof(RenderArgs), hipMemcpyHostToDevice, stream));
+    return NGI_OK;
+}
+
+template <typename T>
+static int dev_alloc(T** ptr, size_t elems) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(elems, 1) * sizeof(T)) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(NGI_ERR_OOM, "batch: hipMalloc of %zu bytes failed", elems * sizeof(T));
+    }
+    *ptr = reinterpret_cast<T*>(p);
+    return NGI_OK;
+}
+
+int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, const ngi_batch_opts& o) {
+    int st = validate(sc, p, count);
+    if (st) return st;
+    opts = o;
+    if (opts.block_size && opts.block_size != 64 && opts.block_size != 128 && opts.block_size != 256)
+        return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", opts.block_size);
+    if (opts.blocks_per_cu < 0) return fail(NGI_ERR_INVALID_ARG, "blocks_per_cu must be >= 0");
+    scenes.assign(sc, sc + count);
+    params.assign(p, p + count);
+    direct = p[0].renderer == NGI_RENDERER_PTDIRECT;
+    nodes_seen.assign(count, nullptr);
+    version_seen.assign(count, ~0ull);
+    args.assign(count, RenderArgs());
+    offset.assign(count + 1, 0);
+    film_off.assign(count + 1, 0);
+    for (int i = 0; i < count; ++i) {
+        const ngi_render_params& q = params[i];
+        RenderArgs& a = args[i];
+        memset(&a, 0, sizeof(a));
+        a.w = q.width, a.h = q.height;
+        a.npix = (uint32_t)q.width * (uint32_t)q.height;
+        a.sampling = q.sampling;
+        a.path_count = q.sampling == NGI_SAMPLING_PER_PIXEL ? (uint64_t)a.npix * (uint64_t)q.spp : (uint64_t)q.num_samples;
+        a.max_nv = q.max_num_vertices;
+        a.rr_depth = q.rr_depth;
+        a.rr_prob_f = (float)q.rr_prob;
+        a.rr_prob_d = q.rr_prob;
+        a.seed = q.seed;
+        a.stream_id = q.stream_id;
+        offset[i + 1] = offset[i] + a.path_count;
+        film_off[i + 1] = film_off[i] + 3 * (size_t)a.npix;
+    }
+    if ((st = refresh())) return st;
+    int dev = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    NGI_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    NGI_HIP_CHECK(hipEventCreate(&ev0));
+    NGI_HIP_CHECK(hipEventCreate(&ev1));
+    if ((st = dev_alloc(&film, film_off.back())) || (st = dev_alloc(&d_args, count)) || (st = dev_alloc(&d_off, count + 1)) ||
+        (st = dev_alloc(&d_tune_off, count + 1)) || (st = dev_alloc(&d_done, count)) || (st = dev_alloc(&d_tdone, count)) ||
+        (st = dev_alloc(&d_misc, MISC_N)))
+        return st;
+    for (int i = 0; i < count; ++i) args[i].film = film + film_off[i];
+    NGI_HIP_CHECK(hipMemcpyAsync(d_args, args.data(), count * sizeof(RenderArgs), hipMemcpyHostToDevice, stream));
+    NGI_HIP_CHECK(hipMemcpyAsync(d_off, offset.data(), (count + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
+    NGI_HIP_CHECK(hipStreamSynchronize(stream));
+    if ((st = autotune())) return st;
+    return opts.use_graph ? capture() : NGI_OK;
+}
+
+// Times every (block size, resident blocks per CU) candidate on a sample of
+// about opts.tune_paths paths taken proportionally from every scene and
+// keeps the fastest.  Fixed choices in opts are kept as given.
+int Batch::autotune() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::vector<int> blocks = opts.block_size ? std::vector<int>{opts.block_size} : std::vector<int>{64, 128, 256};
+    const unsigned long long total = offset.back();
+    const double f = std::min(1.0, (double)std::max<int64_t>(opts.tune_paths, 1) / (double)std::max<unsigned long long>(total, 1));
+    std::vector<unsigned long long> toff(n() + 1, 0);
+    for (int i = 0; i < n(); ++i) toff[i + 1] = toff[i] + (unsigned long long)((double)(offset[i + 1] - offset[i]) * f);
+    NGI_HIP_CHECK(hipMemcpyAsync(d_tune_off, toff.data(), (n() + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
+    struct Cand {
+        int block, per_cu;
+    };
+    std::vector<Cand> cands;
+    for (int bs : blocks) {
+        const int occ = max_per_cu(bs);
+        if (occ <= 0) continue;
+        if (opts.blocks_per_cu) {
+            cands.push_back({bs, std::min(opts.blocks_per_cu, occ)});
+            continue;
+        }
+        for (int per : {occ, (3 * occ + 3) / 4, (occ + 1) / 2}) {
+            per = std::max(1, per);
+            bool seen = false;
+            for (const Cand& c : cands) seen |= c.block == bs && c.per_cu == per;
+            if (!seen) cands.push_back({bs, per});
+        }
+    }
+    if (cands.empty()) return fail(NGI_ERR_DEVICE, "no batch kernel configuration fits on this device");
+    int best = 0;
+    if (cands.size() > 1 && toff.back() > 0) {
+        float best_ms = 0.0f;
+        for (size_t c = 0; c < cands.size(); ++c) {
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; ++rep) {   // the first run warms caches and loads the code
+                int st = enqueue_clear();
+                if (st) return st;
+                NGI_HIP_CHECK(hipEventRecord(ev0, stream));
+                if ((st = enqueue_render(d_tune_off, toff.back(), cands[c].block, cands[c].per_cu))) return st;
+                NGI_HIP_CHECK(hipEventRecord(ev1, stream));
+                NGI_HIP_CHECK(hipEventSynchronize(ev1));
+                NGI_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            }
+            if (c == 0 || ms < best_ms) best_ms = ms, best = (int)c;
+        }
+    }
+    opts.block_size = cands[best].block;
+    opts.blocks_per_cu = cands[best].per_cu;
+    tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return NGI_OK;
+}
+
+int Batch::capture() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr, graph = nullptr;
+    NGI_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_clear();
+    if (!st) st = enqueue_render(d_off, offset.back(), opts.block_size, opts.blocks_per_cu);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(NGI_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    graph = g;
+    NGI_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return NGI_OK;
+}
+
+int Batch::render(uint32_t f) {
+    frame = f;
+    int st = refresh();
+    if (st) return st;
+    NGI_HIP_CHECK(hipEventRecord(ev0, stream));
+    if (exec) {
+        NGI_HIP_CHECK(hipGraphLaunch(exec, stream));
+    } else {
+        if ((st = enqueue_clear())) return st;
+        if ((st = enqueue_render(d_off, offset.back(), opts.block_size, opts.blocks_per_cu))) return st;
+    }
+    NGI_HIP_CHECK(hipEventRecord(ev1, stream));
+    rendered = true;
+    return NGI_OK;
+}
+
+int Batch::sync() {
+    NGI_HIP_CHECK(hipStreamSynchronize(stream));
+    return NGI_OK;
+}
+
+int Batch::get_film(int i, double* out) {
+    if (i < 0 || i >= n()) return fail(NGI_ERR_INVALID_ARG, "scene index %d out of range", i);
+    if (!rendered) return fail(NGI_ERR_INVALID_ARG, "the batch has not been rendered");
+    const size_t m = film_off[i + 1] - film_off[i];
+    NGI_HIP_CHECK(hipMemcpyAsync(out, film + film_off[i], m * sizeof(double), hipMemcpyDeviceToHost, stream));
+    NGI_HIP_CHECK(hipStreamSynchronize(stream));
+    // W*H/N scaling (src/nanogi.cpp:434-437).
+    const double scale = args[i].path_count ? (double)args[i].npix / (double)args[i].path_count : 0.0;
+    for (size_t k = 0; k < m; ++k) out[k] *= scale;
+    return NGI_OK;
+}
+
+int Batch::get_times(double* scene_ms, ngi_stats* stats) {
+    if (!rendered) return fail(NGI_ERR_INVALID_ARG, "the batch has not been rendered");
+    NGI_HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<unsigned long long> td(n()), misc(MISC_N);
+    NGI_HIP_CHECK(hipMemcpy(td.data(), d_tdone, n() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    NGI_HIP_CHECK(hipMemcpy(misc.data(), d_misc, MISC_N * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (scene_ms)
+        for (int i = 0; i < n(); ++i)
+            scene_ms[i] = (td[i] && td[i] >= misc[MISC_T_START] && rate_khz > 0) ? (double)(td[i] - misc[MISC_T_START]) / (double)rate_khz : 0.0;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        float ms = 0.0f;
+        NGI_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats->samples = (int64_t)offset.back();
+        stats->extension_rays = (int64_t)misc[MISC_EXT];
+        stats->shadow_rays = (int64_t)misc[MISC_SHADOW];
+        stats->ms_total = ms;
+        stats->ms_shade = ms;
+        stats->trace_launches = 1;
+    }
+    return NGI_OK;
+}
+
+void Batch::destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* q : {(void*)film, (void*)d_args, (void*)d_off, (void*)d_tune_off, (void*)d_done, (void*)d_tdone, (void*)d_misc})
+        if (q) (void)hipFree(q);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+    exec = nullptr, graph = nullptr, stream = nullptr, ev0 = ev1 = nullptr;
+    film = nullptr, d_args = nullptr, d_off = d_tune_off = d_done = d_tdone = d_misc = nullptr;
+}
+
+}  // namespace ngi
+
+struct ngi_batch : ngi::Batch {};
+
+using namespace ngi;
+
+extern "C" {
+
+void ngi_batch_opts_default(ngi_batch_opts* o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->use_graph = 1;
+    o->tune_paths = 1ll << 24;
+}
+
+int ngi_batch_create(ngi_scene* const* scenes, const ngi_render_params* params, int num_scenes, const ngi_batch_opts* opts, ngi_batch** out) {
+    clear_error();
+    if (!scenes || !params || !out || num_scenes <= 0) return fail(NGI_ERR_INVALID_ARG, "null argument or no scenes");
+    *out = nullptr;
+    ngi_batch_opts o;
+    if (opts) o = *opts;
+    else ngi_batch_opts_default(&o);
+    ngi_batch* b = new ngi_batch();
+    const int st = b->create(scenes, params, num_scenes, o);
+    if (st) {
+        b->destroy();
+        delete b;
+        return st;
+    }
+    *out = b;
+    return NGI_OK;
+}
+
+int ngi_batch_render(ngi_batch* b, uint32_t frame) {
+    clear_error();
+    if (!b) return fail(NGI_ERR_INVALID_ARG, "null batch");
+    return b->render(frame);
+}
+
+int ngi_batch_sync(ngi_batch* b) {
+    clear_error();
+    if (!b) return fail(NGI_ERR_INVALID_ARG, "null batch");
+    return b->sync();
+}
+
+int ngi_batch_get_film(ngi_batch* b, int scene, double* film_out) {
+    clear_error();
+    if (!b || !film_out) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    return b->get_film(scene, film_out);
+}
+
+int ngi_batch_get_times(ngi_batch* b, double* scene_ms, ngi_stats* stats) {
+    clear_error();
+    if (!b) return fail(NGI_ERR_INVALID_ARG, "null batch");
+    return b->get_times(scene_ms, stats);
+}
+
+int ngi_batch_get_opts(ngi_batch* b, ngi_batch_opts* chosen, double* tune_ms) {
+    clear_error();
+    if (!b || !chosen) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    *chosen = b->opts;
+    if (tune_ms) *tune_ms = b->tune_ms;
+    return NGI_OK;
+}
+
+void ngi_batch_destroy(ngi_batch* b) {
+    if (!b) return;
+    b->destroy();
+    delete b;
+}
+
+int ngi_render_batch(ngi_scene* const* scenes, const ngi_render_params* params, int num_scenes, double* const* films, double* scene_ms,
+                     ngi_stats* stats) {
+    clear_error();
+    if (!scenes || !params || !films || num_scenes <= 0) return fail(NGI_ERR_INVALID_ARG, "null argument or no scenes");
+    int st = validate(scenes, params, num_scenes);
+    if (st) return st;
+    // One launch: no tuning trials and no graph.
+    ngi_batch_opts o;
+    ngi_batch_opts_default(&o);
+    o.block_size = 256;
+    o.blocks_per_cu = 1 << 20;   // clamped to the occupancy limit
+    o.use_graph = 0;
+    ngi_batch* b = nullptr;
+    if ((st = ngi_batch_create(scenes, params, num_scenes, &o, &b))) return st;
+    const auto t0 = std::chrono::steady_clock::now();
+    st = b->render(0);
+    if (!st) st = b->sync();
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; !st && i < num_scenes; ++i) st = b->get_film(i, films[i]);
+    if (!st) st = b->get_times(scene_ms, stats);
+    if (!st && stats) stats->ms_total = wall;
+    const std::string err = st ? std::string(ngi_last_error()) : std::string();
+    ngi_batch_destroy(b);
+    if (st) fail(st, "%s", err.c_str());
+    return st;
+}
+
+}  // extern "C"

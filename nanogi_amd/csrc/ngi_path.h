@@ -160,4 +160,96 @@ __device__ __forceinline__ V3<R> camera_sample(const RenderArgs& a, const Cam<R>
     return camera_dir(cam, rx, ry);
 }
 
+// Per-lane path state between vertices of ProcessSample_PT/_PTDirect.
+template <typename R>
+struct PathState {
+    V3<R> T;          // throughput
+    Surf<R> s;        // current vertex
+    V3<R> wi;         // direction towards the previous vertex
+    int type, prim, nv, pixel;
+    uint32_t c0, c1;  // RNG counter words of this path
+};
+
+// Camera vertex of path `id` (SampleEmitter(E) + SamplePosition, src/nanogi.cpp:448-480).
+template <typename R>
+__device__ __forceinline__ void path_start(const RenderArgs& a, uint64_t id, const Cam<R>& cam, PathState<R>& ps) {
+    path_coords(a, id, ps.c0, ps.c1, ps.pixel);
+    ps.T = V3<R>(R(1));
+    ps.type = NGI_PRIM_E;
+    ps.prim = a.sc.sensor_prim;
+    ps.s.p = cam.eye;
+    ps.s.degenerate = true;
+    ps.s.gn = V3<R>(R(0));
+    ps.s.f.n = ps.s.f.s = ps.s.f.t = V3<R>(R(0));
+    ps.wi = V3<R>(R(0));
+    ps.nv = 1;
+}
+
+// One iteration of the vertex loop (src/nanogi.cpp:483-606 / 645-801):
+// NEE (ptdirect), direction sample, extension ray, emission (pt), Russian
+// roulette.  Returns false when the path ends.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+__device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
+                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt) {
+    const DevScene& sc = a.sc;
+    if (ps.nv >= max_nv) return false;
+    if (DIRECT) {
+        const U4 rn = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_NEE, a.seed, a.stream_id);
+        TraceRay sr;
+        int target;
+        const V3<R> C = nee_contribution<R>(sc, cam, prims, ps.type, ps.prim, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
+        if (!is_zero(C)) {
+            TraceHit sh;
+            TraceCounters tc{0, 0};
+            const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, stk, stride);
+            cnt[CNT_SHADOW_RAYS]++;
+            if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+            if (!occluded) film_add<FilmT>(a.film, target, C);
+        }
+    }
+    const U4 rx = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_EXT, a.seed, a.stream_id);
+    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
+    V3<R> wo;
+    if (ps.type == NGI_PRIM_E) {
+        wo = camera_sample<R>(a, cam, u0, u1, ps.pixel);
+    } else {
+        V3<R> wgt;
+        if (!bsdf_sample<R>(prims[ps.prim], ps.type, ps.s, ps.wi, u0, u1, uc, wo, wgt)) return false;
+        ps.T = ps.T * wgt;
+    }
+    TraceRay er;
+    er.ox = (float)ps.s.p.x, er.oy = (float)ps.s.p.y, er.oz = (float)ps.s.p.z;
+    er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
+    er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
+    TraceHit h;
+    TraceCounters tc{0, 0};
+    const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, stk, stride);
+    cnt[CNT_EXT_RAYS]++;
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    if (!hit) return false;
+    int hp;
+    const Surf<R> ns = hit_surface<R>(sc, ps.s.p, wo, h, hp);
+    const DevPrim<R>& hm = prims[hp];
+    if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
+        // Emission on hit (src/nanogi.cpp:568-575): area lights only.
+        const DevLight<R>& L = lights_of<R>(sc)[hm.light];
+        if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -wo) > R(0))
+            film_add<FilmT>(a.film, ps.pixel, ps.T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
+    }
+    if (a.rr_depth >= 0 && ps.nv >= a.rr_depth) {   // src/nanogi.cpp:583-591
+        const R q = rr_prob<R>(a);
+        if (urr > q) return false;
+        ps.T = ps.T / q;
+    }
+    ps.type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    if (ps.type == 0) return false;   // [L]-only primitives absorb (rt.hpp:1146-1147)
+    ps.s = ns;
+    ps.prim = hp;
+    ps.wi = -wo;
+    ps.nv++;
+    return true;
+}
+
+__device__ __forceinline__ int clamp_vertices(int m) { return (m < 0 || m > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : m; }
+
 }  // namespace ngi

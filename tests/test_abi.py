@@ -34,7 +34,7 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof from a C compiler against the ctypes mirror."""
     structs = {"ngi_primitive": ngi.Primitive, "ngi_scene_desc": ngi.SceneDesc, "ngi_bvh_opts": ngi.BvhOpts,
                "ngi_bvh_info": ngi.BvhInfo, "ngi_ray": ngi.Ray, "ngi_hit": ngi.Hit,
-               "ngi_render_params": ngi.RenderParams, "ngi_stats": ngi.Stats}
+               "ngi_render_params": ngi.RenderParams, "ngi_stats": ngi.Stats, "ngi_batch_opts": ngi.BatchOpts}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -112,3 +112,35 @@ def test_no_gpu_reports_device_error():
         ngi.render(s, width=8, height=8, spp=1, sampling=ngi.SAMPLING_PER_PIXEL)
     assert e.value.status == ngi.ERR_DEVICE
     assert "no HIP device" in e.value.message
+
+
+def test_batch_arguments_are_checked_before_the_device():
+    a = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    b = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 2, 1.0, 200)
+    ok = dict(width=8, height=8, spp=1, sampling=ngi.SAMPLING_PER_PIXEL)
+    cases = [
+        ([dict(ok, renderer="pt"), dict(ok, renderer="ptdirect")], ngi.ERR_UNSUPPORTED),
+        ([dict(ok), dict(ok, precision=ngi.PRECISION_FP64_ALL)], ngi.ERR_UNSUPPORTED),
+        ([dict(ok), dict(ok, renderer="bdpt")], ngi.ERR_UNSUPPORTED),
+        ([dict(ok), dict(ok, width=0)], ngi.ERR_INVALID_ARG),
+        ([dict(ok), dict(ok, spp=0)], ngi.ERR_INVALID_ARG),
+        ([dict(ok), dict(ok, rr_depth=1, rr_prob=2.0)], ngi.ERR_INVALID_ARG),
+    ]
+    for kws, status in cases:
+        with pytest.raises(ngi.NgiError) as e:
+            ngi.render_batch([a, b], [ngi.default_params(**k) for k in kws])
+        assert e.value.status == status, kws
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.render_batch([], [])
+    assert e.value.status == ngi.ERR_INVALID_ARG
+    with pytest.raises(ValueError):
+        ngi.render_batch([a], [])
+    # The reusable batch validates the same way, and its options too.
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.Batch([a, b], [ngi.default_params(**dict(ok, renderer="pt")), ngi.default_params(**dict(ok, renderer="ptdirect"))])
+    assert e.value.status == ngi.ERR_UNSUPPORTED
+    o = ngi.batch_opts()
+    assert (o.block_size, o.blocks_per_cu, o.use_graph, o.tune_paths) == (0, 0, 1, 1 << 24)
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.Batch([a], [ngi.default_params(**ok)], ngi.batch_opts(block_size=100))
+    assert e.value.status == ngi.ERR_INVALID_ARG

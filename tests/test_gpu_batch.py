@@ -1,0 +1,138 @@
+"""Batched rendering (ngi_render_batch) against one render per scene.
+
+A scene's paths in a batch are the paths ngi_render draws for the same
+parameters (same ids, seeds and stream ids), traced by the same vertex loop,
+so each batch film equals the single-scene film up to the order of the FP64
+film additions.  The single-scene renders are themselves pinned against the
+oracle by test_gpu_render.py; one small scene is also checked against the
+oracle directly here.
+"""
+import numpy as np
+import pytest
+
+import nanogi_amd as ngi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _p(renderer, w, h, spp, m=8, rr=3, seed=1, stream_id=0, sampling=ngi.SAMPLING_PER_PIXEL, num_samples=0):
+    p = ngi.default_params(renderer=renderer, width=w, height=h, spp=spp, max_num_vertices=m, rr_depth=rr, seed=seed,
+                           sampling=sampling, engine=ngi.ENGINE_PERSISTENT)
+    p.stream_id = stream_id
+    if num_samples:
+        p.num_samples = num_samples
+    return p
+
+
+def _same(a, b):
+    """Same paths, FP64 film sums in a different order: pixels agree to 1e-9
+    relative.  A lane's FP32 arithmetic may be scheduled differently in the
+    batch kernel, so up to 1% of pixels may differ by a path that took another
+    branch at a grazing hit; the image means still agree to 2e-3."""
+    err = np.abs(a - b).max(axis=2)
+    scale = np.abs(b).max(axis=2) + 1e-6 * np.abs(b).max()
+    frac = (err <= 1e-9 * scale).mean()
+    assert frac >= 0.99, frac
+    np.testing.assert_allclose(a.reshape(-1, 3).mean(0), b.reshape(-1, 3).mean(0), rtol=2e-3)
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_batch_films_equal_single_renders(renderer):
+    scenes = [ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0),
+              ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 7, 1.5, 3000),
+              ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 8, 0.75, 40),
+              ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 3, 1.0, 4)]
+    params = [_p(renderer, 33, 29, 5, seed=3, stream_id=0),
+              _p(renderer, 48, 32, 2, m=12, rr=4, seed=4, stream_id=1),
+              _p(renderer, 16, 24, 17, m=-1, rr=1, seed=5, stream_id=2),
+              _p(renderer, 40, 40, 1, seed=6, stream_id=3, sampling=ngi.SAMPLING_REFERENCE_UNIFORM, num_samples=4000)]
+    films, ms, st = ngi.render_batch(scenes, params)
+    assert st["samples"] == 33 * 29 * 5 + 48 * 32 * 2 + 16 * 24 * 17 + 4000
+    assert np.all(ms > 0) and np.all(ms <= st["ms_total"] + 1.0)
+    for s, p, f in zip(scenes, params, films):
+        ref, _ = ngi.render(s, p)
+        assert f.shape == ref.shape and np.abs(ref).max() > 0
+        _same(f, ref)
+
+
+def test_batch_matches_oracle():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _p("ptdirect", 24, 24, 8, m=9, rr=-1, seed=9)
+    films, _, _ = ngi.render_batch([s, s], [p, p])
+    o = oracle.render(s, p)
+    _same(films[0], films[1])
+    err = np.abs(films[0] - o).max(axis=2)
+    scale = np.abs(o).max(axis=2) + 1e-3 * np.abs(o).max()
+    assert (err <= 2e-4 * scale).mean() >= 0.98
+    np.testing.assert_allclose(films[0].reshape(-1, 3).mean(0), o.reshape(-1, 3).mean(0), rtol=2e-3)
+
+
+def test_batch_of_many_small_scenes_and_binary_bvh():
+    scenes, params = [], []
+    rng = np.random.default_rng(0)
+    for i in range(37):
+        s = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 100 + i, 1.0, int(rng.integers(12, 600)))
+        s.build(ngi.BVH_2)
+        scenes.append(s)
+        params.append(_p("pt", int(rng.integers(4, 20)), int(rng.integers(4, 20)), int(rng.integers(1, 6)), seed=i, stream_id=i))
+    films, ms, _ = ngi.render_batch(scenes, params)
+    for k in (0, 17, 36):
+        _same(films[k], ngi.render(scenes[k], params[k])[0])
+    assert np.all(ms > 0)
+
+
+def test_batch_rejects_mixed_bvh_widths():
+    a = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    b = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    a.build(ngi.BVH_2)
+    b.build(ngi.BVH_8)
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.render_batch([a, b], [_p("pt", 8, 8, 1)] * 2)
+    assert e.value.status == ngi.ERR_INVALID_ARG
+
+
+def test_batch_graph_replays_frames_and_follows_scene_updates():
+    """Each frame replays the captured graph (fill kernel + persistent launch);
+    eager renders run between replays, and every replay is checked."""
+    scenes = [ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 20 + i, 1.0, n) for i, n in enumerate((30, 900, 4000))]
+    params = [_p("ptdirect", 24 + 8 * i, 20, 3 + i, seed=10 + i, stream_id=i) for i in range(3)]
+    b = ngi.Batch(scenes, params)
+    cfg = b.config()
+    assert cfg["block_size"] in (64, 128, 256) and cfg["blocks_per_cu"] >= 1 and cfg["use_graph"] == 1
+    for frame in (0, 1, 0, 2):
+        b.render(frame)
+        b.sync()
+        for i, (s, p) in enumerate(zip(scenes, params)):
+            q = ngi.default_params()
+            ctypes_copy(q, p)
+            q.seed = p.seed + frame
+            _same(b.film(i), ngi.render(s, q)[0])
+        ms, st = b.times()
+        assert np.all(ms > 0) and st["samples"] == sum(p.width * p.height * p.spp for p in params)
+        assert st["extension_rays"] >= st["samples"]
+    # Changed reflectances are picked up by the next frame.
+    d = scenes[1].desc()["primitives"]
+    refl = np.where((d["type"] & ngi.PRIM_D)[:, None] > 0, d["D_R"], d["G_R"]) * 0.5
+    scenes[1].set_params(refl, d["Le"])
+    b.render(0)
+    b.sync()
+    _same(b.film(1), ngi.render(scenes[1], params[1])[0])
+    b.close()
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_batch_fixed_launch_shapes_and_plain_launches(block):
+    scenes = [ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0), ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 3, 1.0, 500)]
+    params = [_p("pt", 20, 20, 4, seed=1), _p("pt", 30, 10, 6, seed=2, stream_id=1)]
+    b = ngi.Batch(scenes, params, ngi.batch_opts(block_size=block, blocks_per_cu=2, use_graph=0))
+    assert b.config()["block_size"] == block and b.config()["blocks_per_cu"] <= 2
+    b.render(0)
+    b.sync()
+    for i in range(2):
+        _same(b.film(i), ngi.render(scenes[i], params[i])[0])
+
+
+def ctypes_copy(dst, src):
+    for name, _ in src._fields_:
+        setattr(dst, name, getattr(src, name))
