@@ -232,6 +232,71 @@ def bench_adjoint(a):
     }
 
 
+def rmse_study(a, scene, params, ngi):
+    """RMSE of each estimator variant at `--rmse-spp` spp against an FP64
+    reference at `--rmse-ref-spp` spp (FP64 shading, persistent engine), the
+    per-pixel z-test against it, and the time each variant needs to reach
+    the FP64 estimator's RMSE (time scales with spp, error with 1/sqrt(spp))."""
+    import numpy as np
+    W, H = a.width, a.height
+    batch = 64
+
+    def run(engine, precision, spp, seed, width=8):
+        if scene.bvh_info()["width"] != width:
+            scene.build(width)
+        se = ngi.Session(scene, params(engine=engine, precision=precision, seed=seed, spp=batch))
+        se.render_batch(batch)
+        se.sync()
+        se.reset()
+        se.sync()
+        t0 = time.perf_counter()
+        for k in range(spp // batch):
+            se.render_batch(batch)
+            if (k + 1) % 32 == 0:
+                log(f"rmse study: {(k + 1) * batch}/{spp} spp")
+        se.sync()
+        dt = time.perf_counter() - t0
+        film, var = se.film(variance=True)
+        se.close()
+        return film, var, dt
+
+    log(f"rmse study: FP64 reference at {a.rmse_ref_spp} spp")
+    ref, ref_var, ref_s = run(ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL, a.rmse_ref_spp, 4242)
+    variants = [("persistent_fp32_fp64acc", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC),
+                ("persistent_fp32_all", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_ALL),
+                ("wavefront_fp32_fp64acc", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC),
+                ("wavefront_fp32_all", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_ALL),
+                ("megakernel_fp32_fp64acc", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC),
+                ("megakernel_fp64", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL),
+                ("persistent_fp64", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL)]
+    out = {"reference": {"spp": a.rmse_ref_spp, "precision": "fp64 shading, fp32 traversal", "seconds": round(ref_s, 2),
+                         "rmse_noise": float(np.sqrt(np.mean(ref_var)))}, "spp": a.rmse_spp, "variants": {}}
+    for name, eng, prec in variants:
+        film, var, dt = run(eng, prec, a.rmse_spp, 17)
+        err = film - ref
+        rmse = float(np.sqrt(np.mean(err * err)))
+        # Remove the reference's own noise from the measured error.
+        rmse_vs_truth = float(np.sqrt(max(np.mean(err * err) - np.mean(ref_var), 0.0)))
+        z = err / np.sqrt(var + ref_var + 1e-300)
+        ok = (var + ref_var) > 0
+        out["variants"][name] = {"rmse": rmse, "rmse_minus_ref_noise": rmse_vs_truth,
+                                 "rmse_from_variance": float(np.sqrt(np.mean(var))),
+                                 "seconds": round(dt, 3), "samples_per_s": round(a.rmse_spp * W * H / dt, 1),
+                                 "z_gt3_frac": float(np.mean(np.abs(z[ok]) > 3.0)), "mean_bias": float(np.mean(err))}
+        log(f"rmse study: {name}: rmse {rmse:.5g} in {dt:.2f} s")
+    target = out["variants"]["persistent_fp64"]["rmse_minus_ref_noise"]
+    out["rmse_target"] = target
+    for name, v in out["variants"].items():
+        e = max(v["rmse_minus_ref_noise"], 1e-30)
+        v["time_to_target_s"] = round(v["seconds"] * (e / target) ** 2, 3)
+    base = out["variants"]["megakernel_fp64"]["time_to_target_s"]
+    for name, v in out["variants"].items():
+        v["speedup_vs_megakernel_fp64"] = round(base / v["time_to_target_s"], 3)
+    if scene.bvh_info()["width"] != a.bvh:
+        scene.build(a.bvh)
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +319,9 @@ def parse():
                     help="scenes of the batched many-scene workload reported beside the main line (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering workload reported beside the main line (0 = skip)")
+    ap.add_argument("--rmse-ref-spp", type=int, default=0,
+                    help="render an FP64 reference at this spp and report each variant's RMSE against it (0 = skip)")
+    ap.add_argument("--rmse-spp", type=int, default=1024, help="spp of each variant in the RMSE study")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -433,6 +501,9 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
 
+    study = None
+    if a.rmse_ref_spp > 0:
+        study = rmse_study(a, scene, params, ngi)
     batch = None
     if a.batch_scenes > 0:
         batch = bench_batch(a)
@@ -460,6 +531,7 @@ def main():
         "bvh_build_s": round(info["build_ms"] / 1e3, 3),
         "batch": batch,
         "adjoint": adjoint,
+        "rmse_study": study,
     }
     print(json.dumps(out), flush=True)
     return 0

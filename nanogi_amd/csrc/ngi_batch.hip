@@ -25,6 +25,11 @@
 
 namespace ngi {
 
+// Minimum waves per SIMD for the batch kernel's register allocation (1 = no constraint).
+#ifndef NGI_BATCH_WAVES
+#define NGI_BATCH_WAVES 1
+#endif
+
 struct BatchDev {
     const RenderArgs* args;           // per scene
     const unsigned long long* offset; // per scene + 1: first global path id
@@ -45,7 +50,7 @@ __device__ __forceinline__ void wave_sum_add(unsigned long long* p, uint32_t v) 
 }
 
 template <bool DIRECT, int WIDTH, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) batch_kernel(BatchDev b) {
+__global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev b) {
     extern __shared__ uint32_t stk[];   // max stack_entries over the scenes x BLOCK (wide BVH only)
     if (threadIdx.x == 0) atomicMin(b.misc + MISC_T_START, (unsigned long long)wall_clock64());
     const uint32_t lane = __lane_id();

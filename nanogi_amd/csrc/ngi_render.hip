@@ -16,6 +16,11 @@
 namespace ngi {
 
 #define NGI_MK_BLOCK 256
+// Minimum waves per SIMD the register allocator must leave room for in the
+// persistent kernel (1 = no constraint).
+#ifndef NGI_PK_WAVES
+#define NGI_PK_WAVES 1
+#endif
 
 template <typename T>
 __device__ __forceinline__ void wave_add(unsigned long long* p, T v) {
@@ -54,7 +59,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
 // next path id (one atomic per wavefront and iteration), so every iteration
 // traces with all lanes busy regardless of how uneven path lengths are.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_MK_BLOCK) persistent_kernel(RenderArgs a, unsigned long long* next) {
+__global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next) {
     extern __shared__ uint32_t stk[];
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     const Cam<R> cam = load_cam<R>(a.sc.cam);

@@ -74,7 +74,9 @@ def test_oracle_adjoint_glossy_and_reference_sampling():
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
 def test_gpu_adjoint_matches_oracle_and_fd(renderer):
     s = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
-    p = _params(renderer, 24, 24, 16, 8, 4)
+    # 256 spp: the few paths whose FP32 rays hit other geometry than the FP64
+    # ones weigh ~1/sqrt(paths) in the gradient; at 16 spp they reach 3.5%.
+    p = _params(renderer, 24, 24, 256, 8, 4)
     p.engine = ngi.ENGINE_PERSISTENT
     g = np.random.default_rng(3).normal(size=(24, 24, 3))
     gR, gL, _ = ngi.render_adjoint(s, p, g)

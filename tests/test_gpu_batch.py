@@ -2,8 +2,8 @@
 
 A scene's paths in a batch are the paths ngi_render draws for the same
 parameters (same ids, seeds and stream ids), traced by the same vertex loop,
-so each batch film equals the single-scene film up to the order of the FP64
-film additions.  The single-scene renders are themselves pinned against the
+so each batch film equals the single-scene film path by path, up to FP32
+rounding and the order of the FP64 film additions.  The single-scene renders are themselves pinned against the
 oracle by test_gpu_render.py; one small scene is also checked against the
 oracle directly here.
 """
@@ -26,15 +26,16 @@ def _p(renderer, w, h, spp, m=8, rr=3, seed=1, stream_id=0, sampling=ngi.SAMPLIN
 
 
 def _same(a, b):
-    """Same paths, FP64 film sums in a different order: pixels agree to 1e-9
-    relative.  A lane's FP32 arithmetic may be scheduled differently in the
-    batch kernel, so up to 1% of pixels may differ by a path that took another
-    branch at a grazing hit; the image means still agree to 2e-3."""
+    """Same paths and random numbers as the single-scene render.  The two
+    kernels are compiled separately, so the compiler may fuse FP32
+    multiply-adds differently and a path that grazes an edge can take
+    another branch: the bar is the one used between the GPU and the oracle,
+    98% of pixels within a relative 2e-4 and image means within 5e-3."""
     err = np.abs(a - b).max(axis=2)
-    scale = np.abs(b).max(axis=2) + 1e-6 * np.abs(b).max()
-    frac = (err <= 1e-9 * scale).mean()
-    assert frac >= 0.99, frac
-    np.testing.assert_allclose(a.reshape(-1, 3).mean(0), b.reshape(-1, 3).mean(0), rtol=2e-3)
+    scale = np.abs(b).max(axis=2) + 1e-3 * np.abs(b).max()
+    frac = (err <= 2e-4 * scale).mean()
+    assert frac >= 0.98, frac
+    np.testing.assert_allclose(a.reshape(-1, 3).mean(0), b.reshape(-1, 3).mean(0), rtol=5e-3)
 
 
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
@@ -61,7 +62,9 @@ def test_batch_matches_oracle():
     p = _p("ptdirect", 24, 24, 8, m=9, rr=-1, seed=9)
     films, _, _ = ngi.render_batch([s, s], [p, p])
     o = oracle.render(s, p)
-    _same(films[0], films[1])
+    # Two copies of one scene in one kernel: identical paths, so only the
+    # order of the FP64 film additions differs.
+    np.testing.assert_allclose(films[0], films[1], rtol=1e-9, atol=1e-12 * np.abs(films[1]).max())
     err = np.abs(films[0] - o).max(axis=2)
     scale = np.abs(o).max(axis=2) + 1e-3 * np.abs(o).max()
     assert (err <= 2e-4 * scale).mean() >= 0.98
