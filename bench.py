@@ -151,8 +151,8 @@ def bench_adjoint(a):
                                   max_num_vertices=17, rr_depth=4, rr_prob=0.5, seed=seed, engine=ngi.ENGINE_PERSISTENT)
 
     target, _ = ngi.render(scene, params(999, 256))
-    R = np.where(is_r[:, None], 0.5, 0.0)
-    L = np.where(is_l[:, None], 10.0, 0.0)
+    R = np.where(is_r[:, None], np.full(3, 0.5), 0.0)
+    L = np.where(is_l[:, None], np.full(3, 10.0), 0.0)
 
     def loss_grad(R, L, seed, adj_seed):
         scene.set_params(R, L)
@@ -227,8 +227,8 @@ def bench_adjoint(a):
         "adam_steps": a.adjoint_steps,
         "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
         "param_rel_err": {"albedo": rel(R, R_true, is_r), "radiance": rel(L, L_true, is_l)},
-        "param_rel_err_start": {"albedo": rel(np.where(is_r[:, None], 0.5, 0.0), R_true, is_r),
-                                "radiance": rel(np.where(is_l[:, None], 10.0, 0.0), L_true, is_l)},
+        "param_rel_err_start": {"albedo": rel(np.where(is_r[:, None], np.full(3, 0.5), 0.0), R_true, is_r),
+                                "radiance": rel(np.where(is_l[:, None], np.full(3, 10.0), 0.0), L_true, is_l)},
     }
 
 

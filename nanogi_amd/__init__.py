@@ -102,7 +102,8 @@ class Stats(ctypes.Structure):
 
 
 class BatchOpts(ctypes.Structure):
-    _fields_ = [("block_size", c_int), ("blocks_per_cu", c_int), ("use_graph", c_int), ("tune_paths", c_int64)]
+    _fields_ = [("block_size", c_int), ("blocks_per_cu", c_int), ("use_graph", c_int), ("chunk_paths", c_int),
+                ("tune_paths", c_int64)]
 
 
 # Every entry point declared in include/ngi.h (checked by the CPU tests).
@@ -491,7 +492,7 @@ class Batch:
         t = c_double()
         _check(lib().ngi_batch_get_opts(self._h, ctypes.byref(o), ctypes.byref(t)))
         return {"block_size": o.block_size, "blocks_per_cu": o.blocks_per_cu, "use_graph": o.use_graph,
-                "tune_paths": o.tune_paths, "tune_ms": t.value}
+                "chunk_paths": o.chunk_paths, "tune_paths": o.tune_paths, "tune_ms": t.value}
 
 
 def save_image(path, film):

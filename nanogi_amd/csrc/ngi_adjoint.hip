@@ -216,6 +216,7 @@ static int adjoint_grid(size_t lds, unsigned& grid) {
 }
 
 int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* dLdI, double* gR_out, double* gLe_out, ngi_stats* stats) {
+    if (is_light_tracer(p.renderer)) return fail(NGI_ERR_UNSUPPORTED, "the adjoint differentiates pt and ptdirect");
     Session S;
     ngi_render_params q = p;
     q.engine = NGI_ENGINE_PERSISTENT;
@@ -228,6 +229,7 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
         S.destroy();
         return fail(NGI_ERR_UNSUPPORTED, "the adjoint shades in FP32 (FP64 is used for the gradient reductions)");
     }
+
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t str = S.stream();
     const uint32_t npix = S.npix;

@@ -1,20 +1,20 @@
 // Batched rendering of many independent scenes in one persistent launch
 // (no reference counterpart: nanogi renders one scene per process).
 //
-// Every scene's paths are numbered consecutively in one global path space
-// (prefix offsets per scene).  Lanes of the persistent kernel take the next
-// global path id as soon as their path ends, find its scene with a binary
-// search over the offsets, and run the same vertex loop as the single-scene
-// engines (path_vertex) on that scene's BVH, film and parameters.  Small and
-// large scenes therefore share the GPU instead of leaving it idle between
-// per-scene launches.  The lane that completes a scene's last path records a
-// device wall-clock time stamp, which gives per-scene completion times.
+// Every scene's paths are cut into chunks numbered scene after scene.  Waves
+// of the persistent kernel take chunks from one counter; their lanes take
+// the chunk's paths as soon as their own path ends and run the same vertex
+// loop as the single-scene engines (path_vertex) on that scene's BVH, film
+// and parameters.  Small and large scenes therefore share the GPU instead
+// of leaving it idle between per-scene launches.  The wave that counts a
+// scene's last path records a device wall-clock time stamp, which gives
+// per-scene completion times.
 //
 // A batch owns its device memory (all films in one buffer, per-scene
 // arguments, counters) and replays one captured hipGraph per frame: a fill
 // kernel that clears films and counters, then the persistent kernel.  The
-// block size and the number of resident blocks per CU are picked by timing
-// candidates on a proportional sample of every scene's paths.
+// block size, the number of resident blocks per CU and the chunk size are
+// picked by timing candidates on a proportional sample of every scene's paths.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -25,23 +25,27 @@
 
 namespace ngi {
 
-// Minimum waves per SIMD for the batch kernel's register allocation (1 = no constraint).
+// Minimum waves per SIMD for the batch kernel's register allocation: 4
+// (at most 128 VGPRs) measured fastest on 256 small scenes (1302 vs 1217 M
+// samples/s unconstrained, 1242 at 5, 1157 at 6).
 #ifndef NGI_BATCH_WAVES
-#define NGI_BATCH_WAVES 1
+#define NGI_BATCH_WAVES 4
 #endif
 
 struct BatchDev {
-    const RenderArgs* args;           // per scene
-    const unsigned long long* offset; // per scene + 1: first global path id
-    unsigned long long* done;         // per scene: completed paths
-    unsigned long long* t_done;       // per scene: wall clock at completion
-    unsigned long long* misc;         // MISC_* words
-    unsigned long long total;
+    const RenderArgs* __restrict__ args;           // per scene
+    const unsigned long long* __restrict__ offset; // per scene + 1: prefix sums of the path counts
+    const unsigned long long* __restrict__ chunk;  // per scene + 1: prefix sums of the chunk counts
+    unsigned long long* done;                      // per scene: completed paths
+    unsigned long long* t_done;                    // per scene: wall clock at completion
+    unsigned long long* misc;                      // MISC_* words
+    unsigned long long num_chunks;
+    uint32_t chunk_paths;                          // paths per chunk (the last chunk of a scene may be shorter)
     int num_scenes;
 };
 
-// misc words: wall clock at start (atomic min), global path counter, ray counters.
-enum { MISC_T_START = 0, MISC_NEXT = 1, MISC_EXT = 2, MISC_SHADOW = 3, MISC_N = 4 };
+// misc words: wall clock at start (atomic min), chunk counter, ray counters.
+enum { MISC_T_START = 0, MISC_CHUNK = 1, MISC_EXT = 2, MISC_SHADOW = 3, MISC_N = 4 };
 
 __device__ __forceinline__ void wave_sum_add(unsigned long long* p, uint32_t v) {
     unsigned long long x = v;
@@ -49,59 +53,110 @@ __device__ __forceinline__ void wave_sum_add(unsigned long long* p, uint32_t v) 
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(p, x);
 }
 
+// Last entry of the prefix array p[0..n] that is <= x (entries repeat for
+// scenes without paths; the last repeat is the scene that owns x).
+__device__ __forceinline__ int owner(const unsigned long long* __restrict__ p, int n, unsigned long long x) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (p[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Scheduling is per wavefront: a wave takes a chunk of consecutive paths of
+// one scene (one atomic on the chunk counter), its lanes take the chunk's
+// paths as their paths end, and the next chunk is taken when this one is
+// handed out.  Chunks are numbered scene after scene, so a wave stays on a
+// scene until the scene's chunks are gone; a chunk of another scene waits
+// until the wave's lanes have finished, so every lane of a wave always
+// works on the same scene and the scene's arguments are wave-uniform.
+// Completed paths are counted per wave and added to the scene's counter
+// when the wave leaves the scene; the addition that completes the count
+// stamps the scene's completion time.
 template <bool DIRECT, int WIDTH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev b) {
     extern __shared__ uint32_t stk[];   // max stack_entries over the scenes x BLOCK (wide BVH only)
     if (threadIdx.x == 0) atomicMin(b.misc + MISC_T_START, (unsigned long long)wall_clock64());
     const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     PathState<float> ps;
-    int si = 0;
-    bool active = false, exhausted = false;
+    bool active = false;
+    // Wave-uniform state.
+    int si = -1;                              // scene of the wave
+    unsigned long long next = 0, end = 0;     // unhanded paths of the current chunk (scene-local ids)
+    int pend_si = -1;                         // a chunk of another scene, taken while lanes were busy
+    unsigned long long pend_next = 0, pend_end = 0;
+    unsigned long long ended = 0;             // paths of scene si this wave finished, not yet counted
+    bool chunks_left = true;
     for (;;) {
-        if (!exhausted) {
-            const unsigned long long need = __ballot(!active);
-            if (need) {
-                const int leader = __ffsll((long long)need) - 1;
-                unsigned long long base = 0;
-                if ((int)lane == leader) base = atomicAdd(b.misc + MISC_NEXT, (unsigned long long)__popcll(need));
-                base = __shfl(base, leader);
-                if (base + __popcll(need) >= b.total) exhausted = true;
-                if (!active) {
-                    const unsigned long long gid = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
-                    if (gid < b.total) {
-                        // Scene of this path: the last offset <= gid (scenes
-                        // with no paths share their offset with the next one).
-                        int lo = 0, hi = b.num_scenes;
-                        while (hi - lo > 1) {
-                            const int mid = (lo + hi) >> 1;
-                            if (b.offset[mid] <= gid) lo = mid;
-                            else hi = mid;
-                        }
-                        si = lo;
-                        const RenderArgs& a = b.args[si];
-                        path_start<float>(a, gid - b.offset[si], load_cam<float>(a.sc.cam), ps);
-                        active = true;
-                    }
+        if (next >= end && chunks_left && pend_si < 0) {
+            unsigned long long k = 0;
+            if (lane == 0) k = atomicAdd(b.misc + MISC_CHUNK, 1ull);
+            k = __shfl(k, 0);
+            if (k >= b.num_chunks) {
+                chunks_left = false;
+            } else {
+                const int cs = owner(b.chunk, b.num_scenes, k);
+                const unsigned long long n = b.offset[cs + 1] - b.offset[cs];
+                const unsigned long long cb = (k - b.chunk[cs]) * (unsigned long long)b.chunk_paths;
+                const unsigned long long ce = cb + b.chunk_paths < n ? cb + b.chunk_paths : n;
+                if (si < 0 || cs == si) {
+                    si = cs;
+                    next = cb, end = ce;
+                } else {
+                    pend_si = cs;
+                    pend_next = cb, pend_end = ce;
                 }
             }
         }
-        if (__ballot(active) == 0) break;
-        if (!active) continue;
-        const RenderArgs& a = b.args[si];
-        active = path_vertex<float, double, DIRECT, WIDTH, false>(a, load_cam<float>(a.sc.cam), prims_of<float>(a.sc), ps, stk + threadIdx.x,
-                                                                  BLOCK, clamp_vertices(a.max_nv), cnt);
-        if (!active) {
-            // The lane that ends a scene's last path stamps its completion time.
-            const unsigned long long n = b.offset[si + 1] - b.offset[si];
-            if (atomicAdd(b.done + si, 1ull) + 1ull == n) b.t_done[si] = (unsigned long long)wall_clock64();
+        const int su = __builtin_amdgcn_readfirstlane(si < 0 ? 0 : si);
+        const RenderArgs& a = b.args[su];
+        if (next < end) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const unsigned long long avail = end - next;
+                const unsigned long long rank = (unsigned long long)__popcll(need & below);
+                if (!active && rank < avail) {
+                    path_start<float>(a, next + rank, load_cam<float>(a.sc.cam), ps);
+                    active = true;
+                }
+                const unsigned long long took = (unsigned long long)__popcll(need);
+                next += took < avail ? took : avail;
+            }
         }
+        if (__ballot(active) == 0) {
+            // The wave has no path in flight: count what it finished on this
+            // scene, then move to the waiting chunk, the next chunk, or stop.
+            if (si >= 0 && ended) {
+                if (lane == 0) {
+                    const unsigned long long n = b.offset[su + 1] - b.offset[su];
+                    if (atomicAdd(b.done + su, ended) + ended == n) b.t_done[su] = (unsigned long long)wall_clock64();
+                }
+                ended = 0;
+            }
+            if (pend_si >= 0) {
+                si = pend_si;
+                next = pend_next, end = pend_end;
+                pend_si = -1;
+                continue;
+            }
+            if (next < end || chunks_left) continue;
+            break;
+        }
+        const bool was = active;
+        if (active)
+            active = path_vertex<float, double, DIRECT, WIDTH, false>(a, load_cam<float>(a.sc.cam), prims_of<float>(a.sc), ps,
+                                                                      stk + threadIdx.x, BLOCK, clamp_vertices(a.max_nv), cnt);
+        ended += (unsigned long long)__popcll(__ballot(was && !active));
     }
     wave_sum_add(b.misc + MISC_EXT, cnt[CNT_EXT_RAYS]);
     wave_sum_add(b.misc + MISC_SHADOW, cnt[CNT_SHADOW_RAYS]);
 }
 
-// Clears the films and per-scene counters and resets the path counter: the
+// Clears the films and per-scene counters and resets the chunk counter: the
 // first node of the captured graph (a kernel, not a memset node).
 __global__ void batch_clear(double* film, size_t film_n, unsigned long long* done, unsigned long long* t_done, int n, unsigned long long* misc) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -146,6 +201,8 @@ struct Batch {
     double* film = nullptr;
     RenderArgs* d_args = nullptr;
     unsigned long long *d_off = nullptr, *d_done = nullptr, *d_tdone = nullptr, *d_misc = nullptr, *d_tune_off = nullptr;
+    unsigned long long *d_chunk = nullptr, *d_tune_chunk = nullptr;   // chunk prefix sums (n + 1)
+    unsigned long long num_chunks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t frame = 0;
@@ -153,14 +210,22 @@ struct Batch {
     double tune_ms = 0.0;
 
     int n() const { return (int)scenes.size(); }
-    BatchDev dev(const unsigned long long* off, unsigned long long total) const {
+    // Chunk prefix sums of the path counts in `off` for chunks of c paths.
+    std::vector<unsigned long long> chunk_table(const std::vector<unsigned long long>& off, uint32_t c) const {
+        std::vector<unsigned long long> ch(off.size(), 0);
+        for (size_t i = 0; i + 1 < off.size(); ++i) ch[i + 1] = ch[i] + (off[i + 1] - off[i] + c - 1) / c;
+        return ch;
+    }
+    BatchDev dev(const unsigned long long* off, const unsigned long long* chunk, unsigned long long nchunks, uint32_t c) const {
         BatchDev b;
         b.args = d_args;
         b.offset = off;
+        b.chunk = chunk;
         b.done = d_done;
         b.t_done = d_tdone;
         b.misc = d_misc;
-        b.total = total;
+        b.num_chunks = nchunks;
+        b.chunk_paths = c;
         b.num_scenes = n();
         return b;
     }
@@ -177,14 +242,18 @@ struct Batch {
         NGI_HIP_CHECK(hipGetLastError());
         return NGI_OK;
     }
-    int enqueue_render(const unsigned long long* off, unsigned long long total, int block, int per_cu) const {
-        if (!total) return NGI_OK;
+    int enqueue_render(const unsigned long long* off, const unsigned long long* chunk, unsigned long long nchunks, uint32_t c, int block,
+                       int per_cu) const {
+        if (!nchunks) return NGI_OK;
         BatchKernel k = pick_kernel(direct, width, block);
         if (!k) return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", block);
         const unsigned grid = (unsigned)std::max(1, num_cus * per_cu);
-        k<<<grid, block, lds(block), stream>>>(dev(off, total));
+        k<<<grid, block, lds(block), stream>>>(dev(off, chunk, nchunks, c));
         NGI_HIP_CHECK(hipGetLastError());
         return NGI_OK;
+    }
+    int enqueue_frame() const {
+        return enqueue_render(d_off, d_chunk, num_chunks, (uint32_t)opts.chunk_paths, opts.block_size, opts.blocks_per_cu);
     }
     int refresh();
     int autotune();
@@ -281,6 +350,7 @@ int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, c
     if (opts.block_size && opts.block_size != 64 && opts.block_size != 128 && opts.block_size != 256)
         return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", opts.block_size);
     if (opts.blocks_per_cu < 0) return fail(NGI_ERR_INVALID_ARG, "blocks_per_cu must be >= 0");
+    if (opts.chunk_paths < 0 || opts.chunk_paths > (1 << 24)) return fail(NGI_ERR_INVALID_ARG, "chunk_paths must be in [0, 2^24]");
     scenes.assign(sc, sc + count);
     params.assign(p, p + count);
     direct = p[0].renderer == NGI_RENDERER_PTDIRECT;
@@ -315,7 +385,8 @@ int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, c
     NGI_HIP_CHECK(hipEventCreate(&ev0));
     NGI_HIP_CHECK(hipEventCreate(&ev1));
     if ((st = dev_alloc(&film, film_off.back())) || (st = dev_alloc(&d_args, count)) || (st = dev_alloc(&d_off, count + 1)) ||
-        (st = dev_alloc(&d_tune_off, count + 1)) || (st = dev_alloc(&d_done, count)) || (st = dev_alloc(&d_tdone, count)) ||
+        (st = dev_alloc(&d_tune_off, count + 1)) || (st = dev_alloc(&d_chunk, count + 1)) || (st = dev_alloc(&d_tune_chunk, count + 1)) ||
+        (st = dev_alloc(&d_done, count)) || (st = dev_alloc(&d_tdone, count)) ||
         (st = dev_alloc(&d_misc, MISC_N)))
         return st;
     for (int i = 0; i < count; ++i) args[i].film = film + film_off[i];
@@ -323,49 +394,60 @@ int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, c
     NGI_HIP_CHECK(hipMemcpyAsync(d_off, offset.data(), (count + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
     NGI_HIP_CHECK(hipStreamSynchronize(stream));
     if ((st = autotune())) return st;
+    const std::vector<unsigned long long> ch = chunk_table(offset, (uint32_t)opts.chunk_paths);
+    num_chunks = ch.back();
+    NGI_HIP_CHECK(hipMemcpyAsync(d_chunk, ch.data(), (count + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
+    NGI_HIP_CHECK(hipStreamSynchronize(stream));
     return opts.use_graph ? capture() : NGI_OK;
 }
 
-// Times every (block size, resident blocks per CU) candidate on a sample of
-// about opts.tune_paths paths taken proportionally from every scene and
-// keeps the fastest.  Fixed choices in opts are kept as given.
+// Times every (block size, resident blocks per CU, chunk size) candidate
+// on a sample of about opts.tune_paths paths taken proportionally from
+// every scene and keeps the fastest.  Choices fixed in opts are kept.
 int Batch::autotune() {
     const auto t0 = std::chrono::steady_clock::now();
     const std::vector<int> blocks = opts.block_size ? std::vector<int>{opts.block_size} : std::vector<int>{64, 128, 256};
+    const std::vector<int> chunks = opts.chunk_paths ? std::vector<int>{opts.chunk_paths} : std::vector<int>{256, 2048};
     const unsigned long long total = offset.back();
     const double f = std::min(1.0, (double)std::max<int64_t>(opts.tune_paths, 1) / (double)std::max<unsigned long long>(total, 1));
     std::vector<unsigned long long> toff(n() + 1, 0);
     for (int i = 0; i < n(); ++i) toff[i + 1] = toff[i] + (unsigned long long)((double)(offset[i + 1] - offset[i]) * f);
     NGI_HIP_CHECK(hipMemcpyAsync(d_tune_off, toff.data(), (n() + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
     struct Cand {
-        int block, per_cu;
+        int block, per_cu, chunk;
     };
     std::vector<Cand> cands;
     for (int bs : blocks) {
         const int occ = max_per_cu(bs);
         if (occ <= 0) continue;
-        if (opts.blocks_per_cu) {
-            cands.push_back({bs, std::min(opts.blocks_per_cu, occ)});
-            continue;
-        }
-        for (int per : {occ, (3 * occ + 3) / 4, (occ + 1) / 2}) {
-            per = std::max(1, per);
-            bool seen = false;
-            for (const Cand& c : cands) seen |= c.block == bs && c.per_cu == per;
-            if (!seen) cands.push_back({bs, per});
-        }
+        std::vector<int> pers;
+        if (opts.blocks_per_cu) pers.push_back(std::min(opts.blocks_per_cu, occ));
+        else
+            for (int per : {occ, (3 * occ + 3) / 4, (occ + 1) / 2})
+                if (std::find(pers.begin(), pers.end(), std::max(1, per)) == pers.end()) pers.push_back(std::max(1, per));
+        for (int per : pers)
+            for (int c : chunks) cands.push_back({bs, per, c});
     }
     if (cands.empty()) return fail(NGI_ERR_DEVICE, "no batch kernel configuration fits on this device");
     int best = 0;
     if (cands.size() > 1 && toff.back() > 0) {
         float best_ms = 0.0f;
+        int loaded_chunk = -1;
+        unsigned long long tchunks = 0;
         for (size_t c = 0; c < cands.size(); ++c) {
+            if (cands[c].chunk != loaded_chunk) {
+                const std::vector<unsigned long long> tch = chunk_table(toff, (uint32_t)cands[c].chunk);
+                tchunks = tch.back();
+                NGI_HIP_CHECK(hipMemcpyAsync(d_tune_chunk, tch.data(), (n() + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
+                NGI_HIP_CHECK(hipStreamSynchronize(stream));
+                loaded_chunk = cands[c].chunk;
+            }
             float ms = 0.0f;
             for (int rep = 0; rep < 2; ++rep) {   // the first run warms caches and loads the code
                 int st = enqueue_clear();
                 if (st) return st;
                 NGI_HIP_CHECK(hipEventRecord(ev0, stream));
-                if ((st = enqueue_render(d_tune_off, toff.back(), cands[c].block, cands[c].per_cu))) return st;
+                if ((st = enqueue_render(d_tune_off, d_tune_chunk, tchunks, (uint32_t)cands[c].chunk, cands[c].block, cands[c].per_cu))) return st;
                 NGI_HIP_CHECK(hipEventRecord(ev1, stream));
                 NGI_HIP_CHECK(hipEventSynchronize(ev1));
                 NGI_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -375,6 +457,7 @@ int Batch::autotune() {
     }
     opts.block_size = cands[best].block;
     opts.blocks_per_cu = cands[best].per_cu;
+    opts.chunk_paths = cands[best].chunk;
     tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return NGI_OK;
 }
@@ -385,7 +468,7 @@ int Batch::capture() {
     exec = nullptr, graph = nullptr;
     NGI_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int st = enqueue_clear();
-    if (!st) st = enqueue_render(d_off, offset.back(), opts.block_size, opts.blocks_per_cu);
+    if (!st) st = enqueue_frame();
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &g);
     if (st) {
@@ -407,7 +490,7 @@ int Batch::render(uint32_t f) {
         NGI_HIP_CHECK(hipGraphLaunch(exec, stream));
     } else {
         if ((st = enqueue_clear())) return st;
-        if ((st = enqueue_render(d_off, offset.back(), opts.block_size, opts.blocks_per_cu))) return st;
+        if ((st = enqueue_frame())) return st;
     }
     NGI_HIP_CHECK(hipEventRecord(ev1, stream));
     rendered = true;
@@ -458,13 +541,14 @@ void Batch::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void* q : {(void*)film, (void*)d_args, (void*)d_off, (void*)d_tune_off, (void*)d_done, (void*)d_tdone, (void*)d_misc})
+    for (void* q : {(void*)film, (void*)d_args, (void*)d_off, (void*)d_tune_off, (void*)d_chunk, (void*)d_tune_chunk, (void*)d_done,
+                    (void*)d_tdone, (void*)d_misc})
         if (q) (void)hipFree(q);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
     exec = nullptr, graph = nullptr, stream = nullptr, ev0 = ev1 = nullptr;
-    film = nullptr, d_args = nullptr, d_off = d_tune_off = d_done = d_tdone = d_misc = nullptr;
+    film = nullptr, d_args = nullptr, d_off = d_tune_off = d_chunk = d_tune_chunk = d_done = d_tdone = d_misc = nullptr;
 }
 
 }  // namespace ngi
@@ -479,6 +563,7 @@ void ngi_batch_opts_default(ngi_batch_opts* o) {
     if (!o) return;
     memset(o, 0, sizeof(*o));
     o->use_graph = 1;
+    o->chunk_paths = 0;
     o->tune_paths = 1ll << 24;
 }
 
@@ -549,6 +634,7 @@ int ngi_render_batch(ngi_scene* const* scenes, const ngi_render_params* params, 
     ngi_batch_opts_default(&o);
     o.block_size = 256;
     o.blocks_per_cu = 1 << 20;   // clamped to the occupancy limit
+    o.chunk_paths = 1024;
     o.use_graph = 0;
     ngi_batch* b = nullptr;
     if ((st = ngi_batch_create(scenes, params, num_scenes, &o, &b))) return st;

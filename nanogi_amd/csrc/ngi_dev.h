@@ -48,9 +48,11 @@ struct DevLight {
     uint32_t tri_count;
     uint32_t cdf_begin;  // first entry in cdf (tri_count + 1 entries)
     uint32_t pad;
-    T inv_area;
+    T inv_area;       // area light: 1 / area; directional: the reference's 1 / (2 pi r^2)
     T Le[3];
-    T pos[3];         // point light position
+    T pos[3];         // point light position; directional: centre of the scene bound
+    T dir[3];         // directional: direction as given (not normalised, as rt.hpp:1861)
+    T radius;         // directional: disk radius (rt.hpp:2071)
 };
 
 struct DevCamera {

@@ -71,6 +71,11 @@ static int upload_params(HostScene& s, DeviceScene& d) {
         L.prim = pi;
         L.type = p.light_type;
         for (int k = 0; k < 3; ++k) L.Le[k] = p.Le[k], L.pos[k] = p.light_position[k];
+        if (p.light_type == NGI_LIGHT_DIRECTIONAL) {
+            for (int k = 0; k < 3; ++k) L.pos[k] = s.dir_center[3 * (size_t)pi + k], L.dir[k] = p.light_direction[k];
+            L.radius = s.dir_radius[pi];
+            L.inv_area = s.dir_inv_area[pi];
+        }
         if (p.light_type == NGI_LIGHT_AREA) {
             L.tri_begin = tri_row;
             L.tri_count = p.tri_count;
@@ -83,7 +88,8 @@ static int upload_params(HostScene& s, DeviceScene& d) {
         DevLight<float>& F = lf[i];
         F.prim = L.prim, F.type = L.type, F.tri_begin = L.tri_begin, F.tri_count = L.tri_count, F.cdf_begin = L.cdf_begin, F.pad = 0;
         F.inv_area = (float)L.inv_area;
-        for (int k = 0; k < 3; ++k) F.Le[k] = (float)L.Le[k], F.pos[k] = (float)L.pos[k];
+        for (int k = 0; k < 3; ++k) F.Le[k] = (float)L.Le[k], F.pos[k] = (float)L.pos[k], F.dir[k] = (float)L.dir[k];
+        F.radius = (float)L.radius;
     }
     if (!d.prim_f_mem) {
         NGI_HIP_CHECK(hipMalloc(&d.prim_f_mem, std::max<size_t>(np, 1) * sizeof(DevPrim<float>)));

@@ -1,0 +1,85 @@
+// Light tracers (lt, ltdirect) on the persistent engine: lanes take the
+// next light-path id when their path ends (one atomic per wavefront), as the
+// path tracers' persistent kernel does.  The vertex loop is in ngi_light.h.
+#include <algorithm>
+
+#include "ngi_light.h"
+#include "ngi_render.h"
+
+namespace ngi {
+
+#define NGI_LT_BLOCK 256
+
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+__global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsigned long long* next) {
+    extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    const Cam<R> cam = load_cam<R>(a.sc.cam);
+    const DevPrim<R>* prims = prims_of<R>(a.sc);
+    const int max_nv = clamp_vertices(a.max_nv);
+    const uint32_t lane = __lane_id();
+    LightState<R> ls;
+    bool active = false, exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= a.path_count) exhausted = true;
+                if (!active) {
+                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    if (local < a.path_count) {
+                        cnt[CNT_SAMPLES]++;
+                        active = light_start<R>(a, a.path_begin + local, ls);
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) active = light_vertex<R, FilmT, DIRECT, WIDTH, COUNT>(a, cam, prims, ls, stk + threadIdx.x, NGI_LT_BLOCK, max_nv, cnt);
+    }
+    if (a.counters) {
+        for (int k = 0; k < CNT_N; ++k) {
+            if (!COUNT && (k == CNT_EXT_NODES || k == CNT_EXT_TRIS || k == CNT_SH_NODES || k == CNT_SH_TRIS)) continue;
+            unsigned long long x = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if ((threadIdx.x & 63) == 0 && x) atomicAdd(a.counters + k, x);
+        }
+    }
+}
+
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+static int launch_lt(const RenderArgs& a, unsigned long long* next, hipStream_t st) {
+    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4 : 0;
+    auto kern = light_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
+    int dev = 0, cus = 0, per_cu = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_LT_BLOCK, lds));
+    const uint64_t need_blocks = (a.path_count + NGI_LT_BLOCK - 1) / NGI_LT_BLOCK;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next);
+    NGI_HIP_CHECK(hipGetLastError());
+    return NGI_OK;
+}
+
+template <typename R, typename FilmT, bool COUNT>
+static int launch_lt3(const RenderArgs& a, unsigned long long* next, bool direct, int width, hipStream_t st) {
+    if (direct) return width == 8 ? launch_lt<R, FilmT, true, 8, COUNT>(a, next, st) : launch_lt<R, FilmT, true, 2, COUNT>(a, next, st);
+    return width == 8 ? launch_lt<R, FilmT, false, 8, COUNT>(a, next, st) : launch_lt<R, FilmT, false, 2, COUNT>(a, next, st);
+}
+
+int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_lt3<double, double, false>(a, next, direct, width, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_lt3<float, float, false>(a, next, direct, width, st);
+    if (count) return launch_lt3<float, double, true>(a, next, direct, width, st);
+    return launch_lt3<float, double, false>(a, next, direct, width, st);
+}
+
+}  // namespace ngi

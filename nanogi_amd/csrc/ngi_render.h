@@ -19,6 +19,10 @@ struct WavefrontState;   // ngi_wavefront.hip
 
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st);
 int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
+// Light tracers (ngi_light.hip): lt (direct = false) and ltdirect.
+int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
+
+inline bool is_light_tracer(int renderer) { return renderer == NGI_RENDERER_LT || renderer == NGI_RENDERER_LTDIRECT; }
 
 struct Session {
     ngi_scene* scene = nullptr;

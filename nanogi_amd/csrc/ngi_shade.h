@@ -2,7 +2,7 @@
 // precision (float or double).
 //
 // Restates the Primitive member functions of include/nanogi/rt.hpp for the
-// EL transport direction used by pt/ptdirect:
+// EL transport direction of pt/ptdirect and the LE direction of lt/ltdirect:
 //   OrthonormalBasis :55-59, CosineSampleHemisphere :87-109,
 //   UniformSampleTriangle :129-133, PixelIndex :135-140,
 //   SamplePosition :483-592, SampleDirection :692-910,
@@ -67,28 +67,44 @@ __device__ __forceinline__ V3<R> to_world(const Frame<R>& f, V3<R> l) { return f
 template <typename R>
 __device__ __forceinline__ V3<R> vnorm(V3<R> v) { return v / Math<R>::sqrt_(dot(v, v)); }
 
-// Shirley-Chiu concentric map then Malley's projection (rt.hpp:87-109).
+// Shirley-Chiu concentric map of [0,1)^2 onto the unit disk (rt.hpp:87-103).
+template <typename R>
+__device__ __forceinline__ void concentric_disk(R u0, R u1, R& sx, R& sy) {
+    const R vx = R(2) * u0 - R(1), vy = R(2) * u1 - R(1);
+    sx = 0, sy = 0;
+    if (vx == R(0) && vy == R(0)) return;
+    R r, th;
+    const R q = R(NGI_PI * 0.25);
+    if (vx > -vy) {
+        if (vx > vy) r = vx, th = q * (vy / vx);
+        else r = vy, th = q * (R(2) - vx / vy);
+    } else {
+        if (vx < vy) r = -vx, th = q * (R(4) + vy / vx);
+        else r = -vy, th = q * (R(6) - vx / vy);
+    }
+    R sn, cs;
+    Math<R>::sincos_(th, &sn, &cs);
+    sx = r * cs, sy = r * sn;
+}
+
+// Malley's projection of the concentric disk (rt.hpp:105-109).
 template <typename R>
 __device__ __forceinline__ V3<R> cosine_hemisphere(R u0, R u1) {
-    typedef Math<R> M;
-    const R vx = R(2) * u0 - R(1), vy = R(2) * u1 - R(1);
-    R sx = 0, sy = 0;
-    if (!(vx == R(0) && vy == R(0))) {
-        R r, th;
-        const R q = R(NGI_PI * 0.25);
-        if (vx > -vy) {
-            if (vx > vy) r = vx, th = q * (vy / vx);
-            else r = vy, th = q * (R(2) - vx / vy);
-        } else {
-            if (vx < vy) r = -vx, th = q * (R(4) + vy / vx);
-            else r = -vy, th = q * (R(6) - vx / vy);
-        }
-        R sn, cs;
-        M::sincos_(th, &sn, &cs);
-        sx = r * cs, sy = r * sn;
-    }
+    R sx, sy;
+    concentric_disk(u0, u1, sx, sy);
     const R z2 = R(1) - sx * sx - sy * sy;
-    return V3<R>(sx, sy, M::sqrt_(z2 > R(0) ? z2 : R(0)));
+    return V3<R>(sx, sy, Math<R>::sqrt_(z2 > R(0) ? z2 : R(0)));
+}
+
+// UniformSampleSphere (rt.hpp:116-122).
+template <typename R>
+__device__ __forceinline__ V3<R> uniform_sphere(R u0, R u1) {
+    const R z = R(1) - R(2) * u0;
+    const R r2 = R(1) - z * z;
+    const R r = Math<R>::sqrt_(r2 > R(0) ? r2 : R(0));
+    R sp, cp;
+    Math<R>::sincos_(R(2.0 * NGI_PI) * u1, &sp, &cp);
+    return V3<R>(r * cp, r * sp, z);
 }
 
 template <typename R>
@@ -98,11 +114,14 @@ struct Surf {
     bool degenerate;   // pinhole / point light
 };
 
-// Shading-normal correction for EL transport (rt.hpp:994-1005).
-template <typename R>
+// Shading-normal correction (rt.hpp:994-1005): zero when a direction's
+// geometric and shading cosines disagree in sign; 1 for EL transport (pt,
+// ptdirect), (wi.Ns)(wo.Ng) / ((wo.Ns)(wi.Ng)) for LE transport (lt, ltdirect).
+template <typename R, bool LE = false>
 __device__ __forceinline__ R sn_correction(const Surf<R>& s, V3<R> wi, V3<R> wo, R li_z, R lo_z) {
-    const R a = dot(wi, s.gn) * li_z, b = dot(wo, s.gn) * lo_z;
-    return (a <= R(0) || b <= R(0)) ? R(0) : R(1);
+    const R wi_ng = dot(wi, s.gn), wo_ng = dot(wo, s.gn);
+    if (wi_ng * li_z <= R(0) || wo_ng * lo_z <= R(0)) return R(0);
+    return LE ? li_z * wo_ng / (lo_z * wi_ng) : R(1);
 }
 
 template <typename R>
@@ -156,14 +175,15 @@ __device__ __forceinline__ R fresnel_dielectric(R cos_i, R etaI, R etaT) {
 template <typename R>
 __device__ __forceinline__ V3<R> prim_R(const DevPrim<R>& m) { return V3<R>(m.R[0], m.R[1], m.R[2]); }
 
-// EvaluateDirection(geom, type, wi, wo, EL, forceDegenerated = false) for a
-// surface vertex (next-event estimation).  Specular components are zero.
-template <typename R>
+// EvaluateDirection(geom, type, wi, wo, EL or LE, forceDegenerated = false)
+// for a surface vertex (next-event estimation, sensor connection).
+// Specular components are zero.
+template <typename R, bool LE = false>
 __device__ __forceinline__ V3<R> bsdf_eval_nee(const DevPrim<R>& m, int type, const Surf<R>& s, V3<R> wi, V3<R> wo) {
     const V3<R> li = to_local(s.f, wi), lo = to_local(s.f, wo);
     if (type & NGI_PRIM_D) {
         if (li.z <= R(0) || lo.z <= R(0)) return V3<R>(R(0));
-        return prim_R(m) * (R(1.0 / NGI_PI) * sn_correction(s, wi, wo, li.z, lo.z));
+        return prim_R(m) * (R(1.0 / NGI_PI) * sn_correction<R, LE>(s, wi, wo, li.z, lo.z));
     }
     if (type & NGI_PRIM_G) {
         if (li.z <= R(0) || lo.z <= R(0)) return V3<R>(R(0));
@@ -171,16 +191,16 @@ __device__ __forceinline__ V3<R> bsdf_eval_nee(const DevPrim<R>& m, int type, co
         const R D = beckmann_d(m, H);
         const R G = shadow_masking(li, lo, H);
         const V3<R> F = fresnel_conductor(m, dot(li, H));
-        const R k = D * G / (R(4) * li.z) / lo.z * sn_correction(s, wi, wo, li.z, lo.z);
+        const R k = D * G / (R(4) * li.z) / lo.z * sn_correction<R, LE>(s, wi, wo, li.z, lo.z);
         return prim_R(m) * F * k;
     }
     return V3<R>(R(0));
 }
 
 // SampleDirection + EvaluateDirection/EvaluateDirectionPDF (forceDegenerated
-// = true) at a surface vertex.  Returns false when the reference would
-// terminate the path (wo left at zero, or fs == 0).
-template <typename R>
+// = true) at a surface vertex, EL or LE transport.  Returns false when the
+// reference would terminate the path (wo left at zero, or fs == 0).
+template <typename R, bool LE = false>
 __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const Surf<R>& s, V3<R> wi, R u0, R u1, R ucomp,
                                             V3<R>& wo, V3<R>& weight) {
     typedef Math<R> M;
@@ -190,7 +210,7 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
         const V3<R> lo = cosine_hemisphere(u0, u1);
         if (lo.z <= R(0)) return false;
         wo = to_world(s.f, lo);
-        const R corr = sn_correction(s, wi, wo, li.z, dot(s.f.n, wo));
+        const R corr = sn_correction<R, LE>(s, wi, wo, li.z, dot(s.f.n, wo));
         weight = prim_R(m) * corr;
         return !is_zero(weight);
     }
@@ -215,14 +235,14 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
         if (beckmann_d(m, H2) <= R(0)) return false;
         const R G = shadow_masking(li, lo2, H2);
         const V3<R> F = fresnel_conductor(m, dot(li, H2));
-        const R corr = sn_correction(s, wi, wo, li.z, lo2.z);
+        const R corr = sn_correction<R, LE>(s, wi, wo, li.z, lo2.z);
         // fs/pdf = R D G F/(4 ci co) / (D hz/(4 (lo.H) co)) = R G F (lo.H)/(ci hz)
         weight = prim_R(m) * F * (G * dot(lo2, H2) / (li.z * H2.z) * corr);
         return !is_zero(weight);
     }
     if (type & NGI_PRIM_S) {
         V3<R> lo;
-        R scale = R(1);   // eta^2 for refraction in EL transport
+        R scale = R(1);   // eta^2 for refraction in EL transport only (rt.hpp:1096, 1131)
         if (m.spec_type == NGI_SPEC_REFLECTION) {
             if (li.z <= R(0)) return false;
             lo = V3<R>(-li.x, -li.y, li.z);
@@ -238,11 +258,11 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
             bool reflect;
             if (m.spec_type == NGI_SPEC_REFRACTION) {
                 reflect = c2t <= R(0);
-                scale = eta * eta;
+                if (!LE) scale = eta * eta;
             } else {
                 const R Fr = fresnel_dielectric(li.z, etaI, etaT);
                 reflect = ucomp <= Fr;
-                if (!reflect) scale = eta * eta;
+                if (!reflect && !LE) scale = eta * eta;
                 // fs/pdf: reflection R Fr corr / Fr, refraction R (1-Fr) corr eta^2 / (1-Fr)
                 if ((reflect && Fr <= R(0)) || (!reflect && Fr >= R(1))) return false;
             }
@@ -256,7 +276,7 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
         wo = to_world(s.f, lo);
         const R lz = dot(s.f.n, wo);
         if (m.spec_type == NGI_SPEC_REFLECTION && (li.z <= R(0) || lz <= R(0))) return false;
-        const R corr = sn_correction(s, wi, wo, li.z, lz);
+        const R corr = sn_correction<R, LE>(s, wi, wo, li.z, lz);
         weight = prim_R(m) * (corr * scale);
         return !is_zero(weight);
     }
@@ -334,35 +354,48 @@ template <typename R> __device__ __forceinline__ const DevPrim<R>* prims_of(cons
 template <> __device__ __forceinline__ const DevPrim<float>* prims_of<float>(const DevScene& sc) { return sc.prims_f; }
 template <> __device__ __forceinline__ const DevPrim<double>* prims_of<double>(const DevScene& sc) { return sc.prims_d; }
 
+// A point on an area light: triangle by Distribution1D::SampleReuse
+// (basic.hpp:469-475: upper_bound - 1, clamped) over the light's area CDF,
+// then UniformSampleTriangle (rt.hpp:129-133) with the reused coordinate;
+// n = the triangle's winding normal (SampleTriangleMesh, rt.hpp:487-524).
+template <typename R>
+__device__ __forceinline__ void area_light_point(const DevScene& sc, const DevLight<R>& L, R u2x, R u2y, V3<R>& p, V3<R>& n) {
+    const R* cdf = cdf_of<R>(sc) + L.cdf_begin;
+    uint32_t lo = 0, hi = L.tri_count + 1;   // search [lo, hi) for the first cdf > u
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > u2x) hi = mid;
+        else lo = mid + 1;
+    }
+    int i = (int)lo - 1;
+    i = i < 0 ? 0 : (i > (int)L.tri_count - 1 ? (int)L.tri_count - 1 : i);
+    const R c0 = cdf[i], c1 = cdf[i + 1];
+    const R ux = (u2x - c0) / (c1 - c0);
+    const R sq = Math<R>::sqrt_(ux > R(0) ? ux : R(0));
+    const R bx = R(1) - sq, by = u2y * sq;
+    const float4* t = sc.light_tris + 3 * (size_t)(L.tri_begin + i);
+    const float4 a = t[0], b = t[1], c = t[2];
+    const V3<R> p1((R)a.x, (R)a.y, (R)a.z), p2((R)b.x, (R)b.y, (R)b.z), p3((R)c.x, (R)c.y, (R)c.z);
+    p = p1 * (R(1) - bx - by) + p2 * bx + p3 * by;
+    n = vnorm(cross(p2 - p1, p3 - p1));
+}
+
+template <typename R>
+__device__ __forceinline__ int pick_light(const DevScene& sc, R ul) {
+    const int n = sc.num_lights;
+    const int li = (int)(ul * (R)n);
+    return li < 0 ? 0 : (li > n - 1 ? n - 1 : li);
+}
+
 template <typename R>
 __device__ __forceinline__ LightSample<R> sample_light(const DevScene& sc, R ul, R u2x, R u2y) {
     LightSample<R> ls;
     ls.valid = false;
     const int n = sc.num_lights;
-    int li = (int)(ul * (R)n);
-    li = li < 0 ? 0 : (li > n - 1 ? n - 1 : li);
-    const DevLight<R> L = lights_of<R>(sc)[li];
+    const DevLight<R> L = lights_of<R>(sc)[pick_light(sc, ul)];
     ls.Le = V3<R>(L.Le[0], L.Le[1], L.Le[2]);
     if (L.type == NGI_LIGHT_AREA) {
-        // Distribution1D::SampleReuse (basic.hpp:469-475): upper_bound - 1, clamped.
-        const R* cdf = cdf_of<R>(sc) + L.cdf_begin;
-        uint32_t lo = 0, hi = L.tri_count + 1;   // search [lo, hi) for the first cdf > u
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cdf[mid] > u2x) hi = mid;
-            else lo = mid + 1;
-        }
-        int i = (int)lo - 1;
-        i = i < 0 ? 0 : (i > (int)L.tri_count - 1 ? (int)L.tri_count - 1 : i);
-        const R c0 = cdf[i], c1 = cdf[i + 1];
-        const R ux = (u2x - c0) / (c1 - c0);
-        const R sq = Math<R>::sqrt_(ux > R(0) ? ux : R(0));
-        const R bx = R(1) - sq, by = u2y * sq;
-        const float4* t = sc.light_tris + 3 * (size_t)(L.tri_begin + i);
-        const float4 a = t[0], b = t[1], c = t[2];
-        const V3<R> p1((R)a.x, (R)a.y, (R)a.z), p2((R)b.x, (R)b.y, (R)b.z), p3((R)c.x, (R)c.y, (R)c.z);
-        ls.p = p1 * (R(1) - bx - by) + p2 * bx + p3 * by;
-        ls.n = vnorm(cross(p2 - p1, p3 - p1));
+        area_light_point(sc, L, u2x, u2y, ls.p, ls.n);
         ls.degenerate = false;
         ls.inv_pdf = (R)n / L.inv_area;
         ls.valid = true;

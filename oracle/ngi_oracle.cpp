@@ -472,6 +472,18 @@ double FresnelDielectric(V li, double etaI, double etaT) {
 
 // SampleDirection (rt.hpp:692-910); wo untouched (zero) where the reference returns early.
 void SampleDirection(const Prim& P, int type, const Geom& g, V wi, double u0, double u1, double uc, V& wo) {
+    if (type & NGI_PRIM_L) {
+        if (P.p.light_type == NGI_LIGHT_AREA) {
+            wo = g.to_world(CosineSampleHemisphere(u0, u1));
+        } else if (P.p.light_type == NGI_LIGHT_POINT) {
+            // UniformSampleSphere (rt.hpp:116-122)
+            const double z = 1.0 - 2.0 * u0, r = std::sqrt(std::max(0.0, 1.0 - z * z)), ph = 2.0 * Pi * u1;
+            wo = V(r * std::cos(ph), r * std::sin(ph), z);
+        } else {
+            wo = vec3(P.p.light_direction);
+        }
+        return;
+    }
     if (type & NGI_PRIM_E) {
         const double rx = 2.0 * u0 - 1.0, ry = 2.0 * u1 - 1.0;
         const double tf = std::tan(P.fov * 0.5);
@@ -517,14 +529,17 @@ void SampleDirection(const Prim& P, int type, const Geom& g, V wi, double u0, do
     }
 }
 
-double ShadingNormalCorrection(const Geom& g, V wi, V wo) {
-    const double a = dot(wi, g.gn) * LocalCos(g.to_local(wi));
-    const double b = dot(wo, g.gn) * LocalCos(g.to_local(wo));
-    return (a <= 0 || b <= 0) ? 0.0 : 1.0;   // EL transport
+// rt.hpp:994-1005: zero when a direction's geometric and shading cosines
+// disagree in sign; 1 for EL transport, (wi.Ns)(wo.Ng) / ((wo.Ns)(wi.Ng)) for LE.
+double ShadingNormalCorrection(const Geom& g, V wi, V wo, bool le) {
+    const double wiNg = dot(wi, g.gn), woNg = dot(wo, g.gn);
+    const double wiNs = LocalCos(g.to_local(wi)), woNs = LocalCos(g.to_local(wo));
+    if (wiNg * wiNs <= 0 || woNg * woNs <= 0) return 0.0;
+    return le ? wiNs * woNg / (woNs * wiNg) : 1.0;
 }
 
-// EvaluateDirection (rt.hpp:912-1148), EL transport.
-V EvaluateDirection(const Prim& P, int type, const Geom& g, V wi, V wo, bool forceDeg) {
+// EvaluateDirection (rt.hpp:912-1148); le selects TransportDirection::LE.
+V EvaluateDirection(const Prim& P, int type, const Geom& g, V wi, V wo, bool forceDeg, bool le = false) {
     if (type & NGI_PRIM_L) {
         if (P.p.light_type == NGI_LIGHT_AREA) return LocalCos(g.to_local(wo)) <= 0 ? V() : vec3(P.p.Le);
         if (P.p.light_type == NGI_LIGHT_POINT) return vec3(P.p.Le);
@@ -540,7 +555,7 @@ V EvaluateDirection(const Prim& P, int type, const Geom& g, V wi, V wo, bool for
         const double W = ic * ic * ic / A;
         return V(W, W, W);
     }
-    const double corr = ShadingNormalCorrection(g, wi, wo);
+    const double corr = ShadingNormalCorrection(g, wi, wo, le);
     const V li = g.to_local(wi), lo = g.to_local(wo);
     if (type & NGI_PRIM_D) {
         if (LocalCos(li) <= 0 || LocalCos(lo) <= 0) return V();
@@ -561,7 +576,7 @@ V EvaluateDirection(const Prim& P, int type, const Geom& g, V wi, V wo, bool for
         }
         double etaI = P.p.S_eta1, etaT = P.p.S_eta2;
         if (LocalCos(li) < 0) std::swap(etaI, etaT);
-        const double eta = etaI / etaT;
+        const double eta = le ? 1.0 : etaI / etaT;   // eta^2 radiance scaling for EL only (rt.hpp:1096, 1131)
         if (P.p.spec_type == NGI_SPEC_REFRACTION) return vec3(P.p.S_R) * (corr * eta * eta);
         const double Fr = FresnelDielectric(li, etaI, etaT);
         if (LocalCos(li) * LocalCos(lo) >= 0) return vec3(P.p.S_R) * (Fr * corr);
@@ -572,6 +587,11 @@ V EvaluateDirection(const Prim& P, int type, const Geom& g, V wi, V wo, bool for
 
 // EvaluateDirectionPDF (rt.hpp:1150-1336), forceDegenerated = true.
 double EvaluateDirectionPDF(const Prim& P, int type, const Geom& g, V wi, V wo) {
+    if (type & NGI_PRIM_L) {
+        if (P.p.light_type == NGI_LIGHT_AREA) return LocalCos(g.to_local(wo)) <= 0 ? 0.0 : InvPi;   // projected solid angle
+        if (P.p.light_type == NGI_LIGHT_POINT) return InvPi * 0.25;
+        return 1.0;
+    }
     if (type & NGI_PRIM_E) {
         double rx, ry;
         if (!RasterPosition(P, wo, rx, ry)) return 0;
@@ -752,6 +772,97 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
     }
 }
 
+// One light path: ProcessSample_LT (direct = false, src/nanogi.cpp:804-934)
+// or ProcessSample_LTDirect (true, :936-1131).  Random numbers: the light
+// choice and its position from stream 1 at vertex 0, direction sampling and
+// Russian roulette from stream 0 at every vertex, as the GPU renderer.
+void process_light_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt) {
+    const bool direct = prm.renderer == NGI_RENDERER_LTDIRECT;
+    const int W = prm.width, H = prm.height;
+    const uint32_t npix = (uint32_t)W * (uint32_t)H;
+    uint32_t c0, c1;
+    if (prm.sampling == NGI_SAMPLING_PER_PIXEL) {
+        c0 = (uint32_t)(id % npix);
+        c1 = (uint32_t)(id / npix);
+    } else {
+        c0 = (uint32_t)id;
+        c1 = (uint32_t)(id >> 32) | 0x80000000u;
+    }
+    auto rng = [&](uint32_t vertex, uint32_t stream) { return philox(c0, c1, vertex, stream, prm.seed, prm.stream_id); };
+    if (s.lights.empty()) return;
+    // SampleEmitter(L) (uniform per light primitive) and SamplePosition.
+    const Q ql = rng(0, 1);
+    const int n = (int)s.lights.size();
+    const int li = std::min(std::max((int)(unif(ql.v[0]) * n), 0), n - 1);
+    const Prim& L = s.prims[s.lights[li]];
+    const double pdfL = 1.0 / n;
+    Geom geom;
+    SamplePosition(s, L, unif(ql.v[1]), unif(ql.v[2]), geom);
+    const double pdfPL = EvaluatePositionPDF(L);
+    V throughput = V(1, 1, 1) * (1.0 / pdfPL / pdfL);   // EvaluatePosition(geomL, true) = 1
+    const Prim* prim = &L;
+    int type = NGI_PRIM_L;
+    V wi;
+    int numVertices = 1;
+    const Prim& E = s.prims[s.sensor];
+    const int maxv = (prm.max_num_vertices < 0 || prm.max_num_vertices > 1024) ? 1024 : prm.max_num_vertices;
+    for (;;) {
+        if (numVertices >= maxv) break;
+        if (direct) {
+            // Sensor connection (src/nanogi.cpp:1027-1079): the single pinhole,
+            // pdfE = 1.  pdfPE is evaluated on the light primitive L, as the
+            // reference does (:1045), and LeP = L->EvaluatePosition(.., true) = 1.
+            Geom geomE;
+            geomE.degenerated = true;
+            geomE.p = E.eye;
+            const double pdfPE = EvaluatePositionPDF(L);
+            const V ppE = normalize(geomE.p - geom.p);
+            const V fsL = EvaluateDirection(*prim, type, geom, wi, ppE, false, true);
+            const V fsE = EvaluateDirection(E, NGI_PRIM_E, geomE, V(), -ppE, false);
+            const double G = GeometryTerm(geom, geomE);
+            const V C = throughput * fsL * fsE * (G / pdfPE);
+            if (!zero(C) && Visible(s, geom.p, geomE.p, &cnt.shadow)) {
+                double rx = 0, ry = 0;
+                RasterPosition(E, -ppE, rx, ry);
+                const int index = PixelIndex(rx, ry, W, H);
+                film[3 * (size_t)index] += C.x;
+                film[3 * (size_t)index + 1] += C.y;
+                film[3 * (size_t)index + 2] += C.z;
+            }
+        }
+        const Q q = rng((uint32_t)numVertices, 0);
+        const double u0 = unif(q.v[0]), u1 = unif(q.v[1]), uc = unif(q.v[2]), urr = unif(q.v[3]);
+        V wo;
+        SampleDirection(*prim, type, geom, wi, u0, u1, uc, wo);
+        const double pdfD = EvaluateDirectionPDF(*prim, type, geom, wi, wo);
+        const V fs = EvaluateDirection(*prim, type, geom, wi, wo, true, true);
+        if (zero(fs)) break;
+        throughput = throughput * fs / pdfD;
+        Isect is;
+        cnt.ext++;
+        if (!Intersect(s, geom.p, wo, is)) break;
+        // A hit on the sensor adds throughput * We * EvaluatePosition(isect.geom,
+        // false) (:897-903); a pinhole's EvaluatePosition without
+        // forceDegenerated is zero (rt.hpp:590-593), so lt adds nothing.
+        if (prm.rr_depth >= 0 && numVertices >= prm.rr_depth) {
+            if (urr > prm.rr_prob) break;
+            throughput = throughput / prm.rr_prob;
+        }
+        const Prim& hp = s.prims[is.prim];
+        geom = is.geom;
+        prim = &hp;
+        type = hp.p.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+        wi = -wo;
+        numVertices++;
+        if (type == 0) break;   // [L]-only surfaces: the next EvaluateDirection is zero
+    }
+}
+
+void sample_path(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt) {
+    if (prm.renderer == NGI_RENDERER_LT || prm.renderer == NGI_RENDERER_LTDIRECT) process_light_sample(s, prm, id, film, cnt);
+    else process_sample(s, prm, id, film, cnt);
+}
+
 }  // namespace
 
 extern "C" {
@@ -779,7 +890,7 @@ int oracle_trace(const ngi_scene_desc* d, const ngi_ray* rays, size_t n, ngi_hit
 int oracle_render_paths(const ngi_scene_desc* d, const ngi_render_params* prm, uint64_t path_begin, uint64_t path_count, double* film,
                         int num_threads, long long* counts, int use_bvh) {
     if (!d || !prm || !film) return NGI_ERR_INVALID_ARG;
-    if (prm->renderer != NGI_RENDERER_PT && prm->renderer != NGI_RENDERER_PTDIRECT) return NGI_ERR_UNSUPPORTED;
+    if (prm->renderer < NGI_RENDERER_PT || prm->renderer > NGI_RENDERER_LTDIRECT) return NGI_ERR_UNSUPPORTED;
     Scene s;
     int st = load(s, d, use_bvh != 0);
     if (st) return st;
@@ -796,7 +907,7 @@ int oracle_render_paths(const ngi_scene_desc* d, const ngi_render_params* prm, u
             const uint64_t b = next.fetch_add(grain);
             if (b >= path_count) break;
             const uint64_t e = std::min(path_count, b + grain);
-            for (uint64_t i = b; i < e; ++i) process_sample(s, *prm, path_begin + i, films[k], cnts[k]);
+            for (uint64_t i = b; i < e; ++i) sample_path(s, *prm, path_begin + i, films[k], cnts[k]);
         }
     };
     std::vector<std::thread> th;
@@ -887,7 +998,7 @@ int oracle_scene_render_paths(void* h, const ngi_render_params* prm, uint64_t pa
             const uint64_t b = next.fetch_add(256);
             if (b >= path_count) break;
             const uint64_t e = std::min(path_count, b + 256);
-            for (uint64_t i = b; i < e; ++i) process_sample(s, *prm, path_begin + i, films[k], cnts[k]);
+            for (uint64_t i = b; i < e; ++i) sample_path(s, *prm, path_begin + i, films[k], cnts[k]);
         }
     };
     std::vector<std::thread> th;
@@ -910,6 +1021,7 @@ extern "C" {
 // same paths as oracle_render (see ngi_adjoint.hip for the estimator).
 int oracle_render_adjoint(const ngi_scene_desc* d, const ngi_render_params* prm, const double* dLdI, double* gR, double* gLe, int num_threads) {
     if (!d || !prm || !dLdI || !gR || !gLe) return NGI_ERR_INVALID_ARG;
+    if (prm->renderer != NGI_RENDERER_PT && prm->renderer != NGI_RENDERER_PTDIRECT) return NGI_ERR_UNSUPPORTED;
     Scene s;
     int st = load(s, d, true);
     if (st) return st;

@@ -88,10 +88,82 @@ def rect_form_factor(a, b, h):
     return 4 * f
 
 
-def light_in_view(Le=(3.0, 2.0, 1.0)):
-    """A square light facing the camera, covering the centre of the view."""
+def light_in_view(Le=(3.0, 2.0, 1.0), half=0.5):
+    """A square light (2 half x 2 half) facing the camera, covering the centre of the view."""
     B = SceneBuilder()
     li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
-    B.rect(li, (0, 0, 0), (0, 0, 1), (1, 0, 0), 0.5, 0.5)
+    B.rect(li, (0, 0, 0), (0, 0, 1), (1, 0, 0), half, half)
     B.camera((0, 0, 3.0), (0, 0, 0), fov=40.0)
     return B.scene()
+
+
+def rect_form_factor_at(x, z, a, b, h):
+    """Form factor from a point (x, 0, z) facing up to the parallel 2a x 2b
+    rectangle centred above the origin at height h (the point's projection
+    inside it): superposition of four corner-aligned rectangles (Howell C-11)."""
+    def corner(X, Y):
+        X, Y = X / h, Y / h
+        return (X / np.sqrt(1 + X * X) * np.arctan(Y / np.sqrt(1 + X * X)) +
+                Y / np.sqrt(1 + Y * Y) * np.arctan(X / np.sqrt(1 + Y * Y))) / (2 * np.pi)
+    return sum(corner(sx, sz) for sx in (a - x, a + x) for sz in (b - z, b + z))
+
+
+def plane_under_light_wide(a=0.5, b=0.3, h=1.0, Le=(4.0, 3.0, 2.0), rho=(0.8, 0.5, 0.3), fov=30.0):
+    """plane_under_light with a camera at half height looking straight down
+    with a wide field of view (its footprint stays under the light)."""
+    B = SceneBuilder()
+    pl = B.prim(type=ngi.PRIM_D, D_R=rho)
+    B.rect(pl, (0, 0, 0), (0, 1, 0), (1, 0, 0), 20.0, 20.0)
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
+    B.rect(li, (0, h, 0), (0, -1, 0), (1, 0, 0), a, b)
+    B.camera((0, 0.5 * h, 0), (0, 0, 0), up=(0, 0, -1), fov=fov)
+    return B.scene()
+
+
+def tilted_normals_box(Le=(6.0, 5.0, 4.0)):
+    """Open box with a diffuse floor and a glossy back wall whose shading
+    normals are tilted away from the geometric ones, one area light (0.4 x
+    0.4) under the ceiling, camera in front: exercises the shading-normal
+    corrections of both transport directions."""
+    B = SceneBuilder()
+    fl = B.prim(type=ngi.PRIM_D, D_R=(0.7, 0.6, 0.5))
+    n_fl = np.array([0.25, 1.0, 0.1])
+    n_fl = n_fl / np.linalg.norm(n_fl)
+    v = [np.array(p, dtype=float) for p in ((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1))]
+    B.tri(fl, v[0], v[3], v[2], n_fl)
+    B.tri(fl, v[0], v[2], v[1], n_fl)
+    wall = B.prim(type=ngi.PRIM_G, G_R=(0.9, 0.8, 0.7), G_eta=(0.2, 0.4, 1.4), G_k=(3.9, 2.4, 1.8), G_roughness=0.3)
+    n_w = np.array([0.2, -0.15, 1.0])
+    n_w = n_w / np.linalg.norm(n_w)
+    w = [np.array(p, dtype=float) for p in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0))]
+    B.tri(wall, w[0], w[1], w[2], n_w)
+    B.tri(wall, w[0], w[2], w[3], n_w)
+    side = B.prim(type=ngi.PRIM_D, D_R=(0.3, 0.5, 0.4))
+    B.rect(side, (0, 0.5, 0.5), (1, 0, 0), (0, 0, 1), 0.5, 0.5)
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
+    B.rect(li, (0.5, 0.95, 0.5), (0, -1, 0), (1, 0, 0), 0.2, 0.2)
+    B.camera((0.5, 0.5, 2.0), (0.5, 0.4, 0.0), fov=50.0)
+    return B.scene()
+
+
+def plane_with_light(kind, h=1.0, Le=(2.0, 1.5, 1.0), rho=(0.8, 0.5, 0.3), fov=20.0, half=20.0):
+    """Diffuse plane y = 0 (2 half x 2 half) lit by a point light at (0, h, 0)
+    or a directional light shining straight down; camera at h/2 looking down."""
+    B = SceneBuilder()
+    pl = B.prim(type=ngi.PRIM_D, D_R=rho)
+    B.rect(pl, (0, 0, 0), (0, 1, 0), (1, 0, 0), half, half)
+    if kind == "point":
+        B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_POINT, Le=Le, light_position=(0, h, 0))
+    else:
+        B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_DIRECTIONAL, Le=Le, light_direction=(0, -1, 0))
+    B.camera((0, 0.5 * h, 0), (0, 0, 0), up=(0, 0, -1), fov=fov)
+    return B.scene()
+
+
+def pixel_plane_points(W, h, fov):
+    """(x, z) of the plane points seen through the pixel centres of the
+    downward camera of plane_under_light_wide / plane_with_light."""
+    t = np.tan(np.radians(fov * 0.5))
+    c = (np.arange(W) + 0.5) / W * 2 - 1
+    X, Z = np.meshgrid(c * t * 0.5 * h, c * t * 0.5 * h)
+    return X.ravel(), Z.ravel()

@@ -71,7 +71,7 @@ def test_default_params_are_reference_defaults():
 
 def test_unsupported_renderers_fail_before_touching_the_device():
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
-    for r in ("lt", "ltdirect", "bdpt", "ptmnee"):
+    for r in ("bdpt", "ptmnee"):
         with pytest.raises(ngi.NgiError) as e:
             ngi.render(s, renderer=r, width=8, height=8)
         assert e.value.status == ngi.ERR_UNSUPPORTED
@@ -99,6 +99,15 @@ def test_scene_without_sensor_is_rejected():
     with pytest.raises(ngi.NgiError) as e:
         ngi.render(s, width=8, height=8)
     assert e.value.status == ngi.ERR_INVALID_SCENE
+
+
+def test_adjoint_rejects_light_tracers():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    for r in ("lt", "ltdirect"):
+        p = ngi.default_params(renderer=r, width=4, height=4, sampling=ngi.SAMPLING_PER_PIXEL, spp=1)
+        with pytest.raises(ngi.NgiError) as e:
+            ngi.render_adjoint(s, p, np.zeros((4, 4, 3)))
+        assert e.value.status == ngi.ERR_UNSUPPORTED
 
 
 def test_no_gpu_reports_device_error():
@@ -140,7 +149,10 @@ def test_batch_arguments_are_checked_before_the_device():
         ngi.Batch([a, b], [ngi.default_params(**dict(ok, renderer="pt")), ngi.default_params(**dict(ok, renderer="ptdirect"))])
     assert e.value.status == ngi.ERR_UNSUPPORTED
     o = ngi.batch_opts()
-    assert (o.block_size, o.blocks_per_cu, o.use_graph, o.tune_paths) == (0, 0, 1, 1 << 24)
+    assert (o.block_size, o.blocks_per_cu, o.use_graph, o.chunk_paths, o.tune_paths) == (0, 0, 1, 0, 1 << 24)
     with pytest.raises(ngi.NgiError) as e:
         ngi.Batch([a], [ngi.default_params(**ok)], ngi.batch_opts(block_size=100))
+    assert e.value.status == ngi.ERR_INVALID_ARG
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.Batch([a], [ngi.default_params(**ok)], ngi.batch_opts(chunk_paths=-1))
     assert e.value.status == ngi.ERR_INVALID_ARG

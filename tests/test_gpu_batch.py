@@ -103,6 +103,7 @@ def test_batch_graph_replays_frames_and_follows_scene_updates():
     b = ngi.Batch(scenes, params)
     cfg = b.config()
     assert cfg["block_size"] in (64, 128, 256) and cfg["blocks_per_cu"] >= 1 and cfg["use_graph"] == 1
+    assert cfg["chunk_paths"] in (256, 2048)
     for frame in (0, 1, 0, 2):
         b.render(frame)
         b.sync()
@@ -128,8 +129,8 @@ def test_batch_graph_replays_frames_and_follows_scene_updates():
 def test_batch_fixed_launch_shapes_and_plain_launches(block):
     scenes = [ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0), ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 3, 1.0, 500)]
     params = [_p("pt", 20, 20, 4, seed=1), _p("pt", 30, 10, 6, seed=2, stream_id=1)]
-    b = ngi.Batch(scenes, params, ngi.batch_opts(block_size=block, blocks_per_cu=2, use_graph=0))
-    assert b.config()["block_size"] == block and b.config()["blocks_per_cu"] <= 2
+    b = ngi.Batch(scenes, params, ngi.batch_opts(block_size=block, blocks_per_cu=2, use_graph=0, chunk_paths=100))
+    assert b.config()["block_size"] == block and b.config()["blocks_per_cu"] <= 2 and b.config()["chunk_paths"] == 100
     b.render(0)
     b.sync()
     for i in range(2):

@@ -226,3 +226,106 @@ def test_oracle_hit_distance_closed_form():
     # tmax excludes hits at or beyond it (Scene::Visible range, rt.hpp:2258).
     rays["tmax"][0] = 2.0
     assert oracle.trace(s, rays[:1], use_bvh=False)["tri"][0] == ngi.MISS
+
+
+# ---------------------------------------------------------------------------
+# Light tracing: ProcessSample_LT / _LTDirect (src/nanogi.cpp:804-1131)
+# ---------------------------------------------------------------------------
+def _lt_params(renderer, w, h, n, m, seed=1, rr_depth=-1):
+    p = _params(renderer, w, h, 1, m, rr_depth=rr_depth, seed=seed, sampling=ngi.SAMPLING_REFERENCE_UNIFORM)
+    p.num_samples = n
+    return p
+
+
+def test_lt_with_a_pinhole_adds_nothing():
+    """A light path that hits the sensor adds We * EvaluatePosition(geom,
+    false), which is zero for a pinhole (rt.hpp:590-593): lt renders black
+    while still tracing its paths."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    film, counts = oracle.render_paths(s, _lt_params("lt", 8, 8, 20000, 6), 0, 20000)
+    assert np.all(film == 0) and counts[0] > 20000 and counts[1] == 0
+
+
+def test_ltdirect_directly_viewed_light_closed_form():
+    """-m 2: only the light vertex connects to the camera.  The footprint of
+    the 2h x 2h light gets radiance Le scaled by the light's area, because the
+    reference divides by the light's position pdf where the sensor's belongs
+    (src/nanogi.cpp:1045; the pinhole's is 1)."""
+    Le, half = np.array([3.0, 2.0, 1.0]), 0.3
+    s = scenes.light_in_view(tuple(Le), half)
+    area = (2 * half) ** 2
+    frac = (half / (3.0 * np.tan(np.radians(20.0)))) ** 2
+    _stat_close(lambda k: oracle.render(s, _lt_params("ltdirect", 16, 16, 20000, 2, seed=k)).reshape(-1, 3).mean(0),
+                area * Le * frac)
+    # Every splat lands inside the footprint, and one path carries the
+    # contribution n_L A Le G We A of its light point exactly.
+    img = oracle.render(s, _lt_params("ltdirect", 16, 16, 5000, 2, seed=3))
+    assert img[0, 0].sum() == 0 and img[8, 8].sum() > 0
+
+
+def test_ltdirect_plane_under_light_closed_form():
+    """-m 3: light -> diffuse plane -> camera.  Pixel expectation
+    A * rho * Le * F(x) with F the point-to-rectangle form factor."""
+    a, b, h, Le, rho = 0.5, 0.3, 1.0, np.array([4.0, 3.0, 2.0]), np.array([0.8, 0.5, 0.3])
+    s = scenes.plane_under_light_wide(a, b, h, tuple(Le), tuple(rho), fov=30.0)
+    W = 8
+    # Plane points seen through each pixel centre (camera at h/2 looking down, up = -z).
+    t = np.tan(np.radians(15.0))
+    c = (np.arange(W) + 0.5) / W * 2 - 1
+    X, Z = np.meshgrid(c * t * 0.5 * h, c * t * 0.5 * h)
+    F = np.mean([scenes.rect_form_factor_at(x, z, a, b, h) for x, z in zip(X.ravel(), Z.ravel())])
+    expect = (2 * a) * (2 * b) * rho * Le * F
+    _, se = _stat_close(lambda k: oracle.render(s, _lt_params("ltdirect", W, W, 400000, 3, seed=k)).reshape(-1, 3).mean(0),
+                        expect)
+    assert np.all(se < 0.03 * expect)
+
+
+@pytest.mark.parametrize("scene", ["tilted", "room"])
+def test_ltdirect_equals_area_times_ptdirect(scene):
+    """Light tracing with LE shading-normal corrections estimates the same
+    image as path tracing with EL ones; with one light, ltdirect's images
+    are that image times the light's area.  4x4 block means, -m 4."""
+    if scene == "tilted":
+        s, area = scenes.tilted_normals_box(), 0.16
+    else:
+        s, area = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 5, 1.0, 1500), 0.09
+    W = 8
+
+    def blocks(img):
+        return img.reshape(4, 2, 4, 2, 3).mean(axis=(1, 3)).ravel()
+
+    pt = [blocks(oracle.render(s, _params("ptdirect", W, W, 2048, 4, seed=k))) for k in range(1, 5)]
+    lt = [blocks(oracle.render(s, _lt_params("ltdirect", W, W, 600000, 4, seed=k))) / area for k in range(1, 5)]
+    d = np.mean(pt, 0) - np.mean(lt, 0)
+    se = np.hypot(np.std(pt, 0, ddof=1), np.std(lt, 0, ddof=1)) / 2
+    assert np.mean(pt) > 0
+    assert np.all(np.abs(d) <= 4.5 * se + 1e-3 * np.mean(pt)), (d, se)
+
+
+def test_point_light_closed_form_both_directions():
+    """Point light of intensity Le at height h over a diffuse plane:
+    radiance rho/pi Le h / d^3.  ptdirect and ltdirect agree (a point light's
+    forced position pdf is 1, so ltdirect has no area factor)."""
+    h, Le, rho, W, fov = 1.0, np.array([2.0, 1.5, 1.0]), np.array([0.8, 0.5, 0.3]), 8, 20.0
+    s = scenes.plane_with_light("point", h, tuple(Le), tuple(rho), fov)
+    X, Z = scenes.pixel_plane_points(W, h, fov)
+    expect = rho / np.pi * Le * np.mean(h / (X * X + Z * Z + h * h) ** 1.5)
+    pt = oracle.render(s, _params("ptdirect", W, W, 64, 3))
+    np.testing.assert_allclose(pt.reshape(-1, 3).mean(0), expect, rtol=2e-3)
+    _stat_close(lambda k: oracle.render(s, _lt_params("ltdirect", W, W, 300000, 3, seed=k)).reshape(-1, 3).mean(0), expect)
+
+
+def test_directional_light_closed_form():
+    """Directional light straight down onto a diffuse plane.  ptdirect gets
+    nothing from it (EvaluateDirection(.., false) is zero, rt.hpp:934-937).
+    ltdirect starts on the reference's disk of radius r (rt.hpp:2067-2073,
+    position pdf 1 / (2 pi r^2)) and divides by that pdf again at the sensor
+    (src/nanogi.cpp:1045): radiance 2 pi r^2 * 2 * rho/pi * Le, the factor 2
+    being the disk's true area pi r^2 against the 2 pi r^2 the pdf uses."""
+    Le, rho, W, half = np.array([2.0, 1.5, 1.0]), np.array([0.8, 0.5, 0.3]), 8, 20.0
+    s = scenes.plane_with_light("directional", 1.0, tuple(Le), tuple(rho), 20.0, half)
+    pt = oracle.render(s, _params("ptdirect", W, W, 4, 3))
+    assert np.all(pt == 0)
+    r = np.sqrt(2) * half * 1.01   # |centre - bound.max| * 1.01, bound = plane
+    expect = 2 * np.pi * r * r * 2 * rho / np.pi * Le
+    _stat_close(lambda k: oracle.render(s, _lt_params("ltdirect", W, W, 400000, 3, seed=k)).reshape(-1, 3).mean(0), expect)
