@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK) adjoint_kernel(RenderArgs a, Ad
                     const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp);
                     const DevPrim<float>& hm = prims[hp];
                     if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA &&
-                        dot(ns.f.n, -wo) > 0.0f)
+                        dot(ns.n, -wo) > 0.0f)
                         contribute(pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
                     if (a.rr_depth >= 0 && nv >= a.rr_depth) {
                         if (u24f(rx.w) > a.rr_prob_f) alive = false;

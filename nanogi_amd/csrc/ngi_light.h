@@ -50,25 +50,26 @@ __device__ __forceinline__ bool light_start(const RenderArgs& a, uint64_t id, Li
         V3<R> gn;
         area_light_point(sc, L, ux, uy, ls.s.p, gn);
         ls.s.gn = gn;
-        ls.s.f = make_frame(gn);
+        ls.s.n = gn;
         ls.s.degenerate = false;
         ls.pe = R(1) / L.inv_area;
     } else if (L.type == NGI_LIGHT_POINT) {
         ls.s.p = V3<R>(L.pos[0], L.pos[1], L.pos[2]);
         ls.s.gn = V3<R>(R(0));
-        ls.s.f.n = ls.s.f.s = ls.s.f.t = V3<R>(R(0));
+        ls.s.n = V3<R>(R(0));
         ls.s.degenerate = true;
         ls.pe = R(1);
     } else {
         // Point on the disk of radius r behind the scene (rt.hpp:549-561).
         const V3<R> d(L.dir[0], L.dir[1], L.dir[2]);
         ls.s.gn = d;
-        ls.s.f = make_frame(d);
+        ls.s.n = d;
         ls.s.degenerate = false;
+        const Frame<R> fr = make_frame(d);
         R dx, dy;
         concentric_disk(ux, uy, dx, dy);
         const V3<R> c(L.pos[0], L.pos[1], L.pos[2]);
-        ls.s.p = c - d * L.radius + (ls.s.f.s * (dx * L.radius) + ls.s.f.t * (dy * L.radius));
+        ls.s.p = c - d * L.radius + (fr.s * (dx * L.radius) + fr.t * (dy * L.radius));
         ls.pe = R(1) / L.inv_area;
     }
     // EvaluatePosition(geomL, true) / pdfPL / pdfL.
@@ -97,7 +98,7 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
         V3<R> fsL;
         if (ls.type == NGI_PRIM_L) {
             // Emitter EvaluateDirection(.., LE, false) (rt.hpp:917-937).
-            if (L.type == NGI_LIGHT_AREA) fsL = dot(ls.s.f.n, ppE) > R(0) ? Le : V3<R>(R(0));
+            if (L.type == NGI_LIGHT_AREA) fsL = dot(ls.s.n, ppE) > R(0) ? Le : V3<R>(R(0));
             else if (L.type == NGI_LIGHT_POINT) fsL = Le;
             else fsL = V3<R>(R(0));
         } else {
@@ -108,7 +109,7 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
             const R We = camera_importance(cam, -ppE, rx, ry);
             if (We > R(0)) {
                 R G = R(1) / d2;   // GeometryTerm (rt.hpp:2364-2374); the pinhole is degenerate
-                if (!ls.s.degenerate) G *= Math<R>::abs_(dot(ls.s.f.n, ppE));
+                if (!ls.s.degenerate) G *= Math<R>::abs_(dot(ls.s.n, ppE));
                 const V3<R> C = ls.T * fsL * (G * We * ls.pe);
                 if (!is_zero(C)) {
                     TraceRay sr;
@@ -133,8 +134,8 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
         // Emitter SampleDirection / EvaluateDirection(.., LE, true) /
         // EvaluateDirectionPDF (rt.hpp:695-716, 917-937, 1154-1171).
         if (L.type == NGI_LIGHT_AREA) {
-            wo = to_world(ls.s.f, cosine_hemisphere(u0, u1));
-            if (dot(ls.s.f.n, wo) <= R(0)) return false;
+            wo = to_world(make_frame(ls.s.n), cosine_hemisphere(u0, u1));
+            if (dot(ls.s.n, wo) <= R(0)) return false;
             ls.T = ls.T * Le * R(NGI_PI);              // Le / (1/pi), projected solid angle
         } else if (L.type == NGI_LIGHT_POINT) {
             wo = uniform_sphere(u0, u1);

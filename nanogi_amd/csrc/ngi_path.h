@@ -86,7 +86,7 @@ __device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R
                V3<R>((R)n2.x, (R)n2.y, (R)n2.z) * (R)h.v;
     sn = vnorm(sn);
     if (Math<R>::isnan_(sn.x) || Math<R>::isnan_(sn.y) || Math<R>::isnan_(sn.z)) sn = s.gn;
-    s.f = make_frame(sn);
+    s.n = sn;
     s.degenerate = false;
     return s;
 }
@@ -118,7 +118,7 @@ __device__ __forceinline__ V3<R> nee_contribution(const DevScene& sc, const Cam<
     }
     if (!ls.degenerate && dot(ls.n, -ppL) <= R(0)) return V3<R>(R(0));   // area emitter back side
     R G = R(1);
-    if (!s.degenerate) G *= Math<R>::abs_(dot(s.f.n, ppL));
+    if (!s.degenerate) G *= Math<R>::abs_(dot(s.n, ppL));
     if (!ls.degenerate) G *= Math<R>::abs_(dot(ls.n, -ppL));
     G /= d2;
     const V3<R> C = T * fsE * ls.Le * (G * ls.inv_pdf);
@@ -129,11 +129,26 @@ __device__ __forceinline__ V3<R> nee_contribution(const DevScene& sc, const Cam<
     return C;
 }
 
+// Order in which per-pixel path ids visit the image: bands of NGI_TILE_ROWS
+// rows, column by column within a band, so the 64 consecutive ids of one
+// wavefront cover an 8 x 8 block (coherent camera rays) instead of a 64 x 1
+// strip.  A bijection of [0, w h) for any image size; the random numbers of
+// a path depend on its pixel and sample only, not on this order.
+#ifndef NGI_TILE_ROWS
+#define NGI_TILE_ROWS 8
+#endif
+__device__ __forceinline__ int pixel_of(uint32_t j, int w, int h) {
+    const uint32_t r0 = j / (NGI_TILE_ROWS * (uint32_t)w) * NGI_TILE_ROWS;
+    const uint32_t hb = (uint32_t)h - r0 < NGI_TILE_ROWS ? (uint32_t)h - r0 : NGI_TILE_ROWS;
+    const uint32_t k = j - r0 * (uint32_t)w;
+    return (int)((r0 + k % hb) * (uint32_t)w + k / hb);
+}
+
 // Per-path sample coordinates: RNG counter words and the pixel (per-pixel
 // sampling) or -1 (pixel decided by the camera sample).
 __device__ __forceinline__ void path_coords(const RenderArgs& a, uint64_t id, uint32_t& c0, uint32_t& c1, int& pixel) {
     if (a.sampling == NGI_SAMPLING_PER_PIXEL) {
-        pixel = (int)(id % a.npix);
+        pixel = pixel_of((uint32_t)(id % a.npix), a.w, a.h);
         c0 = (uint32_t)pixel;
         c1 = a.sample_base + (uint32_t)(id / a.npix);
     } else {
@@ -180,7 +195,7 @@ __device__ __forceinline__ void path_start(const RenderArgs& a, uint64_t id, con
     ps.s.p = cam.eye;
     ps.s.degenerate = true;
     ps.s.gn = V3<R>(R(0));
-    ps.s.f.n = ps.s.f.s = ps.s.f.t = V3<R>(R(0));
+    ps.s.n = V3<R>(R(0));
     ps.wi = V3<R>(R(0));
     ps.nv = 1;
 }
@@ -233,7 +248,7 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
     if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
         // Emission on hit (src/nanogi.cpp:568-575): area lights only.
         const DevLight<R>& L = lights_of<R>(sc)[hm.light];
-        if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -wo) > R(0))
+        if (L.type == NGI_LIGHT_AREA && dot(ns.n, -wo) > R(0))
             film_add<FilmT>(a.film, ps.pixel, ps.T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
     }
     if (a.rr_depth >= 0 && ps.nv >= a.rr_depth) {   // src/nanogi.cpp:583-591

@@ -107,10 +107,14 @@ __device__ __forceinline__ V3<R> uniform_sphere(R u0, R u1) {
     return V3<R>(r * cp, r * sp, z);
 }
 
+// A path vertex.  Only the shading normal is kept between vertices; the
+// tangent frame (ComputeTangentSpace, rt.hpp:295-300) is rebuilt from it
+// where a BSDF needs local coordinates, which keeps six fewer registers live
+// across every ray traversal.
 template <typename R>
 struct Surf {
     V3<R> p, gn;
-    Frame<R> f;        // f.n = shading normal
+    V3<R> n;           // shading normal
     bool degenerate;   // pinhole / point light
 };
 
@@ -180,7 +184,8 @@ __device__ __forceinline__ V3<R> prim_R(const DevPrim<R>& m) { return V3<R>(m.R[
 // Specular components are zero.
 template <typename R, bool LE = false>
 __device__ __forceinline__ V3<R> bsdf_eval_nee(const DevPrim<R>& m, int type, const Surf<R>& s, V3<R> wi, V3<R> wo) {
-    const V3<R> li = to_local(s.f, wi), lo = to_local(s.f, wo);
+    const Frame<R> fr = make_frame(s.n);
+    const V3<R> li = to_local(fr, wi), lo = to_local(fr, wo);
     if (type & NGI_PRIM_D) {
         if (li.z <= R(0) || lo.z <= R(0)) return V3<R>(R(0));
         return prim_R(m) * (R(1.0 / NGI_PI) * sn_correction<R, LE>(s, wi, wo, li.z, lo.z));
@@ -204,13 +209,14 @@ template <typename R, bool LE = false>
 __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const Surf<R>& s, V3<R> wi, R u0, R u1, R ucomp,
                                             V3<R>& wo, V3<R>& weight) {
     typedef Math<R> M;
-    const V3<R> li = to_local(s.f, wi);
+    const Frame<R> fr = make_frame(s.n);
+    const V3<R> li = to_local(fr, wi);
     if (type & NGI_PRIM_D) {
         if (li.z <= R(0)) return false;
         const V3<R> lo = cosine_hemisphere(u0, u1);
         if (lo.z <= R(0)) return false;
-        wo = to_world(s.f, lo);
-        const R corr = sn_correction<R, LE>(s, wi, wo, li.z, dot(s.f.n, wo));
+        wo = to_world(fr, lo);
+        const R corr = sn_correction<R, LE>(s, wi, wo, li.z, dot(s.n, wo));
         weight = prim_R(m) * corr;
         return !is_zero(weight);
     }
@@ -228,8 +234,8 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
         const R d = dot(-li, H);
         const V3<R> lo = -li - H * (R(2) * d);
         if (lo.z <= R(0)) return false;
-        wo = to_world(s.f, lo);
-        const V3<R> lo2 = to_local(s.f, wo);
+        wo = to_world(fr, lo);
+        const V3<R> lo2 = to_local(fr, wo);
         if (lo2.z <= R(0)) return false;
         const V3<R> H2 = vnorm(li + lo2);
         if (beckmann_d(m, H2) <= R(0)) return false;
@@ -273,8 +279,8 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
                 lo = V3<R>(-eta * li.x, -eta * li.y, ct);
             }
         }
-        wo = to_world(s.f, lo);
-        const R lz = dot(s.f.n, wo);
+        wo = to_world(fr, lo);
+        const R lz = dot(s.n, wo);
         if (m.spec_type == NGI_SPEC_REFLECTION && (li.z <= R(0) || lz <= R(0))) return false;
         const R corr = sn_correction<R, LE>(s, wi, wo, li.z, lz);
         weight = prim_R(m) * (corr * scale);

@@ -115,9 +115,6 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, float4 a4, float4 b4, 
 #ifndef NGI_PAIR_TRIS
 #define NGI_PAIR_TRIS 1      // fetch leaf triangles two at a time
 #endif
-#ifndef NGI_PREFETCH_REFS
-#define NGI_PREFETCH_REFS 0  // fetch child references together with the node boxes (spills; slower)
-#endif
 
 #define NGI_SLAB_SCALE 1.00000036f   // 1 + 2*gamma(3), gamma(n) = n*eps/(1-n*eps), eps = 2^-24
 
@@ -241,61 +238,55 @@ __device__ bool trace_bvh2(const DevScene& sc, const TraceRay& ray, TraceHit& h,
 
 __device__ __forceinline__ float exp2_bits(uint32_t biased) { return __uint_as_float(biased << 23); }
 
-struct Refs8 {
-    int32_t c0, c1, c2, c3, c4, c5, c6, c7;
-};
-
-__device__ __forceinline__ int32_t pick_ref(const Refs8& r, uint32_t slot) {
-    const int32_t lo = (slot & 2u) ? ((slot & 1u) ? r.c3 : r.c2) : ((slot & 1u) ? r.c1 : r.c0);
-    const int32_t hi = (slot & 2u) ? ((slot & 1u) ? r.c7 : r.c6) : ((slot & 1u) ? r.c5 : r.c4);
-    return (slot & 4u) ? hi : lo;
-}
-
-// Box test of the eight children; the child references are fetched in the
-// same batch of loads so the next step does not wait on a second load.
+// Box test of the eight children of `node`: a mask of the hit children in
+// processing order (bit i = slot i ^ octant).
+//
+// The near and far planes of every child are chosen once per node from the
+// sign of the inverse direction (for 1/d >= 0 the low bound is near): with
+// t(q) = fma(q, a, b) and a >= 0, q_lo <= q_hi gives t(q_lo) <= t(q_hi)
+// because rounding is monotone, so this equals min/max of the two planes
+// bit for bit.  The hit bits are built in slot order with constant shifts,
+// masked by the valid slots once, and permuted to processing order at the end.
 template <bool COUNT>
 __device__ __forceinline__ uint32_t visit8(const DevScene& sc, const RayPre& r, uint32_t node, float tmin, float tmax,
-                                           TraceCounters& cnt, Refs8& refs) {
+                                           TraceCounters& cnt) {
     const float4* n = sc.nodes + 8 * (size_t)node;
     const float4 h0 = n[0];
     const uint4 q0 = __builtin_bit_cast(uint4, n[1]);  // qlox[8], qloy[8]
     const uint4 q1 = __builtin_bit_cast(uint4, n[2]);  // qloz[8], qhix[8]
     const uint4 q2 = __builtin_bit_cast(uint4, n[3]);  // qhiy[8], qhiz[8]
-#if NGI_PREFETCH_REFS
-    const int4 ra = __builtin_bit_cast(int4, n[4]), rb = __builtin_bit_cast(int4, n[5]);
-    refs.c0 = ra.x, refs.c1 = ra.y, refs.c2 = ra.z, refs.c3 = ra.w;
-    refs.c4 = rb.x, refs.c5 = rb.y, refs.c6 = rb.z, refs.c7 = rb.w;
-#else
-    (void)refs;
-#endif
     if (COUNT) cnt.nodes++;
     const uint32_t meta = __float_as_uint(h0.w);
     const float scx = exp2_bits(meta & 0xffu), scy = exp2_bits((meta >> 8) & 0xffu), scz = exp2_bits((meta >> 16) & 0xffu);
-    const uint32_t valid = meta >> 24;
     // t(q) = (origin + q * scale - o) * invd = q * (scale * invd) + (origin - o) * invd
     const float ax = scx * r.ix, bx = (h0.x - r.ox) * r.ix;
     const float ay = scy * r.iy, by = (h0.y - r.oy) * r.iy;
     const float az = scz * r.iz, bz = (h0.z - r.oz) * r.iz;
-    uint32_t mask = 0;
+    const bool fx = r.ix < 0.0f, fy = r.iy < 0.0f, fz = r.iz < 0.0f;
+    // Near / far plane words for children 0-3 (.x) and 4-7 (.y).
+    const uint2 nx = fx ? make_uint2(q1.z, q1.w) : make_uint2(q0.x, q0.y), ex = fx ? make_uint2(q0.x, q0.y) : make_uint2(q1.z, q1.w);
+    const uint2 ny = fy ? make_uint2(q2.x, q2.y) : make_uint2(q0.z, q0.w), ey = fy ? make_uint2(q0.z, q0.w) : make_uint2(q2.x, q2.y);
+    const uint2 nz = fz ? make_uint2(q2.z, q2.w) : make_uint2(q1.x, q1.y), ez = fz ? make_uint2(q1.x, q1.y) : make_uint2(q2.z, q2.w);
+    uint32_t hits = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const uint32_t wlox = k < 4 ? q0.x : q0.y, wloy = k < 4 ? q0.z : q0.w;
-        const uint32_t wloz = k < 4 ? q1.x : q1.y, whix = k < 4 ? q1.z : q1.w;
-        const uint32_t whiy = k < 4 ? q2.x : q2.y, whiz = k < 4 ? q2.z : q2.w;
         const int sh = (k & 3) * 8;
-        const float lx = (float)((wlox >> sh) & 0xffu), hx = (float)((whix >> sh) & 0xffu);
-        const float ly = (float)((wloy >> sh) & 0xffu), hy = (float)((whiy >> sh) & 0xffu);
-        const float lz = (float)((wloz >> sh) & 0xffu), hz = (float)((whiz >> sh) & 0xffu);
-        const float tx0 = lx * ax + bx, tx1 = hx * ax + bx;
-        const float ty0 = ly * ay + by, ty1 = hy * ay + by;
-        const float tz0 = lz * az + bz, tz1 = hz * az + bz;
-        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-        const float tfa = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * NGI_SLAB_SCALE;
-        const bool hit = tn <= fminf(tfa, tmax) && ((valid >> k) & 1u);
-        // processing index i = slot ^ octant
-        mask |= (hit ? 1u : 0u) << (k ^ r.octant);
+        const uint32_t wnx = k < 4 ? nx.x : nx.y, wny = k < 4 ? ny.x : ny.y, wnz = k < 4 ? nz.x : nz.y;
+        const uint32_t wex = k < 4 ? ex.x : ex.y, wey = k < 4 ? ey.x : ey.y, wez = k < 4 ? ez.x : ez.y;
+        const float tnx = (float)((wnx >> sh) & 0xffu) * ax + bx, tfx = (float)((wex >> sh) & 0xffu) * ax + bx;
+        const float tny = (float)((wny >> sh) & 0xffu) * ay + by, tfy = (float)((wey >> sh) & 0xffu) * ay + by;
+        const float tnz = (float)((wnz >> sh) & 0xffu) * az + bz, tfz = (float)((wez >> sh) & 0xffu) * az + bz;
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        const float tf = fminf(fminf(tfx, tfy), tfz) * NGI_SLAB_SCALE;
+        if (tn <= fminf(tf, tmax)) hits |= 1u << k;
     }
-    return mask;
+    hits &= meta >> 24;   // valid slots
+    // Slot order -> processing order: swap bit groups selected by the octant.
+    const uint32_t o = r.octant;
+    if (o & 1u) hits = ((hits & 0x55u) << 1) | ((hits >> 1) & 0x55u);
+    if (o & 2u) hits = ((hits & 0x33u) << 2) | ((hits >> 2) & 0x33u);
+    if (o & 4u) hits = ((hits & 0x0fu) << 4) | ((hits >> 4) & 0x0fu);
+    return hits;
 }
 
 template <bool ANY, bool COUNT>
@@ -307,9 +298,7 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
     h.leaf = -1;
     int sp = 0;
     uint32_t cur = 0;
-    Refs8 refs;
-    uint32_t mask = visit8<COUNT>(sc, r, 0, tmin, tmax, cnt, refs);
-    bool fresh = true;   // refs hold the child references of `cur`
+    uint32_t mask = visit8<COUNT>(sc, r, 0, tmin, tmax, cnt);
     bool done = false;
     for (;;) {
         if (mask == 0) {
@@ -317,22 +306,16 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
             const uint32_t e = lds_stack[(--sp) * lane_stride];
             cur = e >> 8;
             mask = e & 0xffu;
-            fresh = false;
             continue;
         }
         const uint32_t i = __builtin_ctz(mask);
         mask &= mask - 1u;
         const uint32_t slot = i ^ r.octant;
-#if NGI_PREFETCH_REFS
-        const int32_t ref = fresh ? pick_ref(refs, slot) : reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
-#else
         const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
-#endif
         if (ref >= 0) {
             if (mask) lds_stack[(sp++) * lane_stride] = (cur << 8) | mask;
             cur = (uint32_t)ref;
-            mask = visit8<COUNT>(sc, r, cur, tmin, tmax, cnt, refs);
-            fresh = true;
+            mask = visit8<COUNT>(sc, r, cur, tmin, tmax, cnt);
         } else {
             leaf_test<ANY, COUNT>(sc, r, ref, tmin, tmax, h, cnt, done);
             if (ANY && done) return true;

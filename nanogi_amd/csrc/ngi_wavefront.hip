@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
             const DevPrim<float>& hm = prims[hp];
             if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
                 const DevLight<float>& L = lights_of<float>(sc)[hm.light];
-                if (L.type == NGI_LIGHT_AREA && dot(ns.f.n, -d) > 0.0f) film_add<FilmT>(a.film, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]));
+                if (L.type == NGI_LIGHT_AREA && dot(ns.n, -d) > 0.0f) film_add<FilmT>(a.film, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]));
             }
             const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
             alive = true;
