@@ -3,7 +3,8 @@
 One step = `--spp-per-step` samples per pixel over the whole image (paths of
 at most 32 segments, Russian roulette from depth 4), rendered into a batch
 film that is folded into per-pixel moments, so the RMSE of the estimate is
-measured from the batch variance.  Prints one JSON line (see README of the
+measured from the batch variance.  The defaults (16 steps of 64 spp) render
+the full 1024-spp image of this workload.  Prints one JSON line (see README of the
 driver contract): samples/s, rays/s, the estimated RMSE at 1024 spp, the
 roofline of the dominant kernel (HBM-bound traversal) and the CPU FP64
 oracle timed on the host cores on a bounded sample of the same workload.
@@ -35,7 +36,7 @@ def measured_traffic(a, engine, precision):
     import nanogi_amd as ngi
     path = os.path.join(ROOT, "profiles", "persistent_ptdirect_sphere_field_1024.md")
     default = (engine == ngi.ENGINE_PERSISTENT and precision == ngi.PRECISION_FP32_FP64ACC and a.bvh == 8 and
-               a.width == 1024 and a.height == 1024 and a.spp_per_step == 4 and a.detail == 8)
+               a.width == 1024 and a.height == 1024 and a.spp_per_step == 64 and a.detail == 8 and a.leaf == 0 and a.ct == 0)
     if not default or not os.path.exists(path):
         return None
     for line in open(path):
@@ -239,7 +240,7 @@ def rmse_study(a, scene, params, ngi):
     the FP64 estimator's RMSE (time scales with spp, error with 1/sqrt(spp))."""
     import numpy as np
     W, H = a.width, a.height
-    batch = 64
+    batch = 16   # spp per batch: the variants' variance comes from rmse_spp / 16 batch means
 
     def run(engine, precision, spp, seed, width=8):
         if scene.bvh_info()["width"] != width:
@@ -279,10 +280,22 @@ def rmse_study(a, scene, params, ngi):
         rmse_vs_truth = float(np.sqrt(max(np.mean(err * err) - np.mean(ref_var), 0.0)))
         z = err / np.sqrt(var + ref_var + 1e-300)
         ok = (var + ref_var) > 0
+        # The same test on 8x8-pixel block means, whose distribution is closer
+        # to normal than single pixels' (paths that hit a light carry heavy tails).
+        # Pixel errors are independent between pixels, so block variances add.
+        bs = lambda x: x[:H // 8 * 8, :W // 8 * 8].reshape(H // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))
+        zb = bs(err) / np.sqrt(bs(var + ref_var) / 64.0 + 1e-300)
+        # |z| > 3 under a t distribution with (batches - 1) degrees of freedom
+        # (the variance comes from a finite number of batch means).
+        from scipy import stats as _st
+        dof = max(a.rmse_spp // batch - 1, 1)
         out["variants"][name] = {"rmse": rmse, "rmse_minus_ref_noise": rmse_vs_truth,
                                  "rmse_from_variance": float(np.sqrt(np.mean(var))),
                                  "seconds": round(dt, 3), "samples_per_s": round(a.rmse_spp * W * H / dt, 1),
-                                 "z_gt3_frac": float(np.mean(np.abs(z[ok]) > 3.0)), "mean_bias": float(np.mean(err))}
+                                 "z_gt3_frac": float(np.mean(np.abs(z[ok]) > 3.0)),
+                                 "z_gt3_frac_blocks8": float(np.mean(np.abs(zb) > 3.0)),
+                                 "z_gt3_expected_t": float(2 * _st.t.sf(3.0, dof)), "batches": a.rmse_spp // batch,
+                                 "mean_bias": float(np.mean(err)), "mean_rel_bias": float(np.mean(err) / np.mean(ref))}
         log(f"rmse study: {name}: rmse {rmse:.5g} in {dt:.2f} s")
     target = out["variants"]["persistent_fp64"]["rmse_minus_ref_noise"]
     out["rmse_target"] = target
@@ -302,7 +315,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp-per-step", type=int, default=4)
+    ap.add_argument("--spp-per-step", type=int, default=64)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--engine", choices=["wavefront", "megakernel", "persistent"], default="persistent")
@@ -311,6 +324,8 @@ def parse():
     ap.add_argument("--precision", choices=["fp32_fp64acc", "fp32_all", "fp64_all"], default="fp32_fp64acc")
     ap.add_argument("--bvh", type=int, choices=[2, 8], default=8)
     ap.add_argument("--detail", type=int, default=8, help="icosphere subdivision levels")
+    ap.add_argument("--leaf", type=int, default=0, help="BVH maximum leaf size (0 = builder default)")
+    ap.add_argument("--ct", type=float, default=0.0, help="BVH SAH node-visit cost (0 = builder default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-counters", action="store_true")
@@ -354,7 +369,7 @@ def main():
     t0 = time.perf_counter()
     scene = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, W / H, a.detail)
     gen_s = time.perf_counter() - t0
-    info = scene.build(a.bvh)
+    info = scene.build(a.bvh, max_leaf_size=a.leaf, traversal_cost=a.ct)
     ntris = scene.num_triangles
     log(f"scene: {ntris} triangles generated in {gen_s:.2f} s; BVH{a.bvh} {info['num_nodes']} nodes, "
         f"depth {info['max_depth']}, built in {info['build_ms'] / 1e3:.2f} s")

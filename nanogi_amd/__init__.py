@@ -61,7 +61,8 @@ class SceneDesc(ctypes.Structure):
 
 
 class BvhOpts(ctypes.Structure):
-    _fields_ = [("width", c_int), ("max_leaf_size", c_int), ("sah_bins", c_int), ("num_threads", c_int)]
+    _fields_ = [("width", c_int), ("max_leaf_size", c_int), ("sah_bins", c_int), ("num_threads", c_int),
+                ("traversal_cost", c_double)]
 
 
 class BvhInfo(ctypes.Structure):
@@ -323,8 +324,8 @@ class Scene:
         _check(lib().ngi_scene_save_yaml(self._h, str(directory).encode()))
 
     # --- acceleration structure --------------------------------------------
-    def build(self, width=BVH_8, max_leaf_size=0, sah_bins=0, num_threads=0):
-        o = BvhOpts(width, max_leaf_size, sah_bins, num_threads)
+    def build(self, width=BVH_8, max_leaf_size=0, sah_bins=0, num_threads=0, traversal_cost=0.0):
+        o = BvhOpts(width, max_leaf_size, sah_bins, num_threads, traversal_cost)
         _check(lib().ngi_scene_build(self._h, ctypes.byref(o)))
         return self.bvh_info()
 

@@ -386,6 +386,8 @@ int bvh_build(const HostScene& s, const ngi_bvh_opts& o, Bvh& out) {
     Ctx c;
     c.max_leaf = o.max_leaf_size > 0 ? std::min(o.max_leaf_size, 16) : (width == NGI_BVH_8 ? 4 : 4);
     c.bins = o.sah_bins > 0 ? std::min(std::max(o.sah_bins, 4), 256) : 32;
+    if (o.traversal_cost < 0 || o.traversal_cost > 1e3) return fail(NGI_ERR_INVALID_ARG, "traversal_cost out of range");
+    if (o.traversal_cost > 0) c.ct = o.traversal_cost;
     int threads = o.num_threads > 0 ? o.num_threads : (int)std::thread::hardware_concurrency();
     c.spare_threads = std::max(threads - 1, 0);
     c.tri_box.resize(nt);

@@ -144,3 +144,15 @@ def test_wide_slots_follow_child_octants():
             agree += octant == k
             total += 1
     assert agree / total > 0.5
+
+
+def test_traversal_cost_option_shapes_the_tree():
+    """A larger SAH node-visit cost makes fewer, larger leaves (up to the leaf cap)."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, 3)
+    cheap = s.build(ngi.BVH_8, max_leaf_size=8, traversal_cost=0.5)
+    dear = s.build(ngi.BVH_8, max_leaf_size=8, traversal_cost=4.0)
+    assert dear["num_leaves"] < cheap["num_leaves"]
+    assert dear["num_triangles"] == cheap["num_triangles"]
+    with pytest.raises(ngi.NgiError) as e:
+        s.build(ngi.BVH_8, traversal_cost=-1.0)
+    assert e.value.status == ngi.ERR_INVALID_ARG
