@@ -858,8 +858,230 @@ void process_light_sample(const Scene& s, const ngi_render_params& prm, uint64_t
     }
 }
 
+// ---------------------------------------------------------------------------
+// Bidirectional path tracing: ProcessSample_BDPT (src/nanogi.cpp:1133-1189)
+// with Path (include/nanogi/bdpt.hpp): SampleSubpath :54-137, Connect
+// :139-191, EvaluateCst :210-244, EvaluateUnweightContribution :246-333,
+// power-heuristic weight (the "Opt" form, every p_i recomputed) :352-372,
+// EvaluatePDF :491-528, SelectionProb :193-208, RasterPosition :210-218.
+// Random numbers: the eye subpath's step k draws stream 0 at vertex k, the
+// light subpath's stream 1 at vertex k (its vertex 0: light choice and
+// position, as the light tracers).
+// ---------------------------------------------------------------------------
+const int kBdptMaxVertices = 64;   // subpath cap; the reference's -m -1 has none
+
+struct PV {
+    int type = 0;
+    Geom geom;
+    const Prim* prim = nullptr;
+};
+
+// EvaluatePosition (rt.hpp:564-620).
+double EvaluatePositionE(const Prim& P, bool forceDeg) {
+    if (P.p.type & NGI_PRIM_L) return P.p.light_type == NGI_LIGHT_POINT ? (forceDeg ? 1.0 : 0.0) : 1.0;
+    return P.p.sensor_type == NGI_SENSOR_PINHOLE ? (forceDeg ? 1.0 : 0.0) : 1.0;
+}
+
+// EvaluateEmitterPDF (rt.hpp:2338-2352): uniform over lights; one sensor.
+double EmitterPDF(const Scene& s, const Prim& P) { return (P.p.type & NGI_PRIM_L) ? 1.0 / (double)s.lights.size() : 1.0; }
+
+double rr_at(const ngi_render_params& prm, int step) { return (prm.rr_depth >= 0 && step >= prm.rr_depth) ? prm.rr_prob : 1.0; }
+
+struct Bdpt {
+    const Scene& s;
+    const ngi_render_params& prm;
+    std::vector<PV> x;   // connected path, light end first
+
+    V dir(int from, int to) const { return normalize(x[to].geom.p - x[from].geom.p); }
+
+    V Cst(int sl) const {
+        const int n = (int)x.size(), t = n - sl;
+        if (sl == 0 && t > 0) {
+            const PV& v = x[0];
+            return EvaluateDirection(*v.prim, v.type, v.geom, V(), dir(0, 1), false) * EvaluatePositionE(*v.prim, false);
+        }
+        if (sl > 0 && t == 0) {
+            const PV& v = x[n - 1];
+            return EvaluateDirection(*v.prim, v.type, v.geom, V(), dir(n - 1, n - 2), false, true) * EvaluatePositionE(*v.prim, false);
+        }
+        const PV& vL = x[sl - 1];
+        const PV& vE = x[sl];
+        const V fsL = EvaluateDirection(*vL.prim, vL.type, vL.geom, sl - 2 >= 0 ? dir(sl - 1, sl - 2) : V(), dir(sl - 1, sl), false, true);
+        const V fsE = EvaluateDirection(*vE.prim, vE.type, vE.geom, sl + 1 < n ? dir(sl, sl + 1) : V(), dir(sl, sl - 1), false);
+        return fsL * fsE * GeometryTerm(vL.geom, vE.geom);
+    }
+
+    V Unweighted(int sl) const {
+        const int n = (int)x.size(), t = n - sl;
+        auto local = [](V f, double p) { return zero(f) ? V() : f / p; };
+        V aL(1, 1, 1), aE(1, 1, 1);
+        if (sl > 0) {
+            const PV& v0 = x[0];
+            aL = local(V(1, 1, 1) * EvaluatePositionE(*v0.prim, true), EvaluatePositionPDF(*v0.prim) * EmitterPDF(s, *v0.prim));
+            for (int i = 0; i < sl - 1; ++i) {
+                const V wi = i >= 1 ? dir(i, i - 1) : V(), wo = dir(i, i + 1);
+                aL = aL * local(EvaluateDirection(*x[i].prim, x[i].type, x[i].geom, wi, wo, true, true),
+                                EvaluateDirectionPDF(*x[i].prim, x[i].type, x[i].geom, wi, wo));
+            }
+        }
+        if (zero(aL)) return V();
+        if (t > 0) {
+            const PV& vn = x[n - 1];
+            aE = local(V(1, 1, 1) * EvaluatePositionE(*vn.prim, true), EvaluatePositionPDF(*vn.prim) * EmitterPDF(s, *vn.prim));
+            for (int i = n - 1; i > sl; --i) {
+                const V wi = i < n - 1 ? dir(i, i + 1) : V(), wo = dir(i, i - 1);
+                aE = aE * local(EvaluateDirection(*x[i].prim, x[i].type, x[i].geom, wi, wo, true),
+                                EvaluateDirectionPDF(*x[i].prim, x[i].type, x[i].geom, wi, wo));
+            }
+        }
+        if (zero(aE)) return V();
+        const V c = Cst(sl);
+        return zero(c) ? V() : aL * c * aE;
+    }
+
+    double PDF(int sl) const {
+        if (zero(Cst(sl))) return 0;
+        const int n = (int)x.size(), t = n - sl;
+        double pdf = 1;
+        if (sl > 0) {
+            pdf *= EvaluatePositionPDF(*x[0].prim) * EmitterPDF(s, *x[0].prim);
+            for (int i = 0; i < sl - 1; ++i) {
+                pdf *= EvaluateDirectionPDF(*x[i].prim, x[i].type, x[i].geom, i >= 1 ? dir(i, i - 1) : V(), dir(i, i + 1));
+                pdf *= GeometryTerm(x[i].geom, x[i + 1].geom);
+            }
+        }
+        if (t > 0) {
+            pdf *= EvaluatePositionPDF(*x[n - 1].prim) * EmitterPDF(s, *x[n - 1].prim);
+            for (int i = n - 1; i >= sl + 1; --i) {
+                pdf *= EvaluateDirectionPDF(*x[i].prim, x[i].type, x[i].geom, i + 1 < n ? dir(i, i + 1) : V(), dir(i, i - 1));
+                pdf *= GeometryTerm(x[i].geom, x[i - 1].geom);
+            }
+        }
+        return pdf;
+    }
+
+    double MIS(int sl) const {
+        const int n = (int)x.size();
+        const double ps = PDF(sl);
+        double inv = 0;
+        for (int i = 0; i <= n; ++i) {
+            const double pi = PDF(i);
+            if (pi > 0) inv += (pi / ps) * (pi / ps);
+        }
+        return 1.0 / inv;
+    }
+
+    double Selection(int sl) const {
+        const int t = (int)x.size() - sl;
+        double p = 1;
+        for (int i = 1; i < sl - 1; ++i) p *= rr_at(prm, i);
+        for (int i = t - 2; i >= 1; --i) p *= rr_at(prm, i);
+        return p;
+    }
+};
+
+void sample_subpath(const Scene& s, const ngi_render_params& prm, uint32_t c0, uint32_t c1, bool le, int pixel, std::vector<PV>& out,
+                    Counts& cnt) {
+    out.clear();
+    const int W = prm.width, H = prm.height;
+    const int maxv = (prm.max_num_vertices < 0 || prm.max_num_vertices > kBdptMaxVertices) ? kBdptMaxVertices : prm.max_num_vertices;
+    const uint32_t stream = le ? 1u : 0u;
+    for (int step = 0; step < maxv; ++step) {
+        const Q q = philox(c0, c1, (uint32_t)step, stream, prm.seed, prm.stream_id);
+        PV v;
+        if (step == 0) {
+            if (le) {
+                const int n = (int)s.lights.size();
+                if (n == 0) return;
+                const int li = std::min(std::max((int)(unif(q.v[0]) * n), 0), n - 1);
+                v.prim = &s.prims[s.lights[li]];
+                v.type = NGI_PRIM_L;
+                SamplePosition(s, *v.prim, unif(q.v[1]), unif(q.v[2]), v.geom);
+            } else {
+                v.prim = &s.prims[s.sensor];
+                v.type = NGI_PRIM_E;
+                v.geom.degenerated = true;
+                v.geom.p = v.prim->eye;
+            }
+            out.push_back(v);
+            continue;
+        }
+        const PV& pv = out.back();
+        const V wi = out.size() > 1 ? normalize(out[out.size() - 2].geom.p - pv.geom.p) : V();
+        const double u0 = unif(q.v[0]), u1 = unif(q.v[1]), uc = unif(q.v[2]), urr = unif(q.v[3]);
+        V wo;
+        if (pv.type == NGI_PRIM_E && pixel >= 0) {
+            // Per-pixel mode: the raster sample lies in this path's pixel.
+            SampleDirection(*pv.prim, pv.type, pv.geom, wi, (pixel % W + u0) / W, (pixel / W + u1) / H, uc, wo);
+        } else {
+            SampleDirection(*pv.prim, pv.type, pv.geom, wi, u0, u1, uc, wo);
+        }
+        if (zero(EvaluateDirection(*pv.prim, pv.type, pv.geom, wi, wo, true, le))) break;
+        Isect is;
+        cnt.ext++;
+        if (!Intersect(s, pv.geom.p, wo, is)) break;
+        v.geom = is.geom;
+        v.prim = &s.prims[is.prim];
+        v.type = v.prim->p.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+        out.push_back(v);
+        const double q_rr = rr_at(prm, step);
+        if (q_rr < 1.0 && urr > q_rr) break;   // the vertex is kept (bdpt.hpp:118-123)
+    }
+}
+
+void process_bdpt_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt) {
+    const int W = prm.width, H = prm.height;
+    const uint32_t npix = (uint32_t)W * (uint32_t)H;
+    uint32_t c0, c1;
+    int pixel = -1;
+    if (prm.sampling == NGI_SAMPLING_PER_PIXEL) {
+        pixel = (int)(id % npix);
+        c0 = (uint32_t)pixel;
+        c1 = (uint32_t)(id / npix);
+    } else {
+        c0 = (uint32_t)id;
+        c1 = (uint32_t)(id >> 32) | 0x80000000u;
+    }
+    std::vector<PV> L, E;
+    sample_subpath(s, prm, c0, c1, true, -1, L, cnt);
+    sample_subpath(s, prm, c0, c1, false, pixel, E, cnt);
+    const int nL = (int)L.size(), nE = (int)E.size();
+    const int maxv = (prm.max_num_vertices < 0 || prm.max_num_vertices > 2 * kBdptMaxVertices) ? 2 * kBdptMaxVertices : prm.max_num_vertices;
+    Bdpt b{s, prm, {}};
+    for (int n = 2; n <= nE + nL; ++n) {
+        if (n > maxv) continue;
+        for (int sl = std::max(0, n - nE); sl <= std::min(nL, n); ++sl) {
+            const int t = n - sl;
+            b.x.clear();
+            if (sl == 0) {
+                if (!(E[t - 1].prim->p.type & NGI_PRIM_L)) continue;
+                for (int i = t - 1; i >= 0; --i) b.x.push_back(E[i]);
+                b.x.front().type = NGI_PRIM_L;
+            } else if (t == 0) {
+                if (!(L[sl - 1].prim->p.type & NGI_PRIM_E)) continue;
+                for (int i = 0; i < sl; ++i) b.x.push_back(L[i]);
+                b.x.back().type = NGI_PRIM_E;
+            } else {
+                if (!Visible(s, L[sl - 1].geom.p, E[t - 1].geom.p, &cnt.shadow)) continue;
+                for (int i = 0; i < sl; ++i) b.x.push_back(L[i]);
+                for (int i = t - 1; i >= 0; --i) b.x.push_back(E[i]);
+            }
+            const V Cu = b.Unweighted(sl);
+            if (zero(Cu)) continue;
+            const V C = Cu * (b.MIS(sl) / b.Selection(sl));
+            double rx = 0, ry = 0;
+            RasterPosition(*b.x[n - 1].prim, b.dir(n - 1, n - 2), rx, ry);
+            const int index = PixelIndex(rx, ry, W, H);
+            film[3 * (size_t)index] += C.x;
+            film[3 * (size_t)index + 1] += C.y;
+            film[3 * (size_t)index + 2] += C.z;
+        }
+    }
+}
+
 void sample_path(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt) {
     if (prm.renderer == NGI_RENDERER_LT || prm.renderer == NGI_RENDERER_LTDIRECT) process_light_sample(s, prm, id, film, cnt);
+    else if (prm.renderer == NGI_RENDERER_BDPT) process_bdpt_sample(s, prm, id, film, cnt);
     else process_sample(s, prm, id, film, cnt);
 }
 
@@ -890,7 +1112,7 @@ int oracle_trace(const ngi_scene_desc* d, const ngi_ray* rays, size_t n, ngi_hit
 int oracle_render_paths(const ngi_scene_desc* d, const ngi_render_params* prm, uint64_t path_begin, uint64_t path_count, double* film,
                         int num_threads, long long* counts, int use_bvh) {
     if (!d || !prm || !film) return NGI_ERR_INVALID_ARG;
-    if (prm->renderer < NGI_RENDERER_PT || prm->renderer > NGI_RENDERER_LTDIRECT) return NGI_ERR_UNSUPPORTED;
+    if (prm->renderer < NGI_RENDERER_PT || prm->renderer > NGI_RENDERER_BDPT) return NGI_ERR_UNSUPPORTED;
     Scene s;
     int st = load(s, d, use_bvh != 0);
     if (st) return st;

@@ -160,10 +160,12 @@ def plane_with_light(kind, h=1.0, Le=(2.0, 1.5, 1.0), rho=(0.8, 0.5, 0.3), fov=2
     return B.scene()
 
 
-def pixel_plane_points(W, h, fov):
-    """(x, z) of the plane points seen through the pixel centres of the
-    downward camera of plane_under_light_wide / plane_with_light."""
+def pixel_plane_points(W, h, fov, sub=1):
+    """(x, z) of the plane points seen through the downward camera of
+    plane_under_light_wide / plane_with_light: `sub` x `sub` evenly spaced
+    raster points per pixel (raster is linear in x and z on the plane, so
+    their mean approximates the pixel average to O(1/sub^2))."""
     t = np.tan(np.radians(fov * 0.5))
-    c = (np.arange(W) + 0.5) / W * 2 - 1
+    c = (np.arange(W * sub) + 0.5) / (W * sub) * 2 - 1
     X, Z = np.meshgrid(c * t * 0.5 * h, c * t * 0.5 * h)
     return X.ravel(), Z.ravel()

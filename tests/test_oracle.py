@@ -329,3 +329,55 @@ def test_directional_light_closed_form():
     r = np.sqrt(2) * half * 1.01   # |centre - bound.max| * 1.01, bound = plane
     expect = 2 * np.pi * r * r * 2 * rho / np.pi * Le
     _stat_close(lambda k: oracle.render(s, _lt_params("ltdirect", W, W, 400000, 3, seed=k)).reshape(-1, 3).mean(0), expect)
+
+
+# ---------------------------------------------------------------------------
+# Bidirectional path tracing: ProcessSample_BDPT (src/nanogi.cpp:1133-1189,
+# include/nanogi/bdpt.hpp).  Unlike ltdirect it has no area factor: every
+# strategy's contribution is divided by the pdf of that strategy, and the
+# power-heuristic weights sum to one over the strategies that can make a path.
+# ---------------------------------------------------------------------------
+def test_bdpt_furnace():
+    Le, rho, m = (1.0, 0.5, 0.25), (0.5, 0.6, 0.7), 4
+    s = scenes.furnace(Le, rho)
+    _stat_close(lambda k: oracle.render(s, _params("bdpt", 4, 4, 256, m, rr_depth=-1, seed=k)).reshape(-1, 3).mean(0),
+                scenes.furnace_radiance(Le, rho, m))
+
+
+def test_bdpt_plane_under_rectangular_light():
+    a, b, h, Le, rho = 0.5, 0.3, 1.0, np.array([4.0, 3.0, 2.0]), np.array([0.8, 0.5, 0.3])
+    s = scenes.plane_under_light(a, b, h, tuple(Le), tuple(rho))
+    _, se = _stat_close(lambda k: oracle.render(s, _params("bdpt", 2, 2, 4096, 3, seed=k)).reshape(-1, 3).mean(0),
+                        rho * Le * scenes.rect_form_factor(a, b, h))
+    assert np.all(se < 0.03 * rho * Le * scenes.rect_form_factor(a, b, h))
+
+
+def test_bdpt_point_and_directional_lights():
+    h, Le, rho, W, fov = 1.0, np.array([2.0, 1.5, 1.0]), np.array([0.8, 0.5, 0.3]), 8, 20.0
+    s = scenes.plane_with_light("point", h, tuple(Le), tuple(rho), fov)
+    X, Z = scenes.pixel_plane_points(W, h, fov, sub=32)
+    expect = rho / np.pi * Le * np.mean(h / (X * X + Z * Z + h * h) ** 1.5)
+    _stat_close(lambda k: oracle.render(s, _params("bdpt", W, W, 64, 3, seed=k)).reshape(-1, 3).mean(0), expect)
+    # Directional: the light subpath starts on the reference's disk, whose
+    # position pdf is 1 / (2 pi r^2) for an area of pi r^2 (rt.hpp:2072).  A
+    # 2 x 2 plane keeps the disk small enough for light subpaths to reach the view.
+    s = scenes.plane_with_light("directional", 1.0, tuple(Le), tuple(rho), 60.0, half=1.0)
+    _stat_close(lambda k: oracle.render(s, _params("bdpt", W, W, 1024, 3, seed=k)).reshape(-1, 3).mean(0), 2 * rho / np.pi * Le)
+
+
+@pytest.mark.parametrize("scene", ["cornell", "tilted", "room"])
+def test_bdpt_equals_ptdirect(scene):
+    """Same expectation as path tracing with NEE, block by block (-m 5, RR from the first bounce)."""
+    s = {"cornell": lambda: ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0), "tilted": scenes.tilted_normals_box,
+         "room": lambda: ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 5, 1.0, 1500)}[scene]()
+    W = 8
+
+    def blocks(img):
+        return img.reshape(4, 2, 4, 2, 3).mean(axis=(1, 3)).ravel()
+
+    pt = [blocks(oracle.render(s, _params("ptdirect", W, W, 2048, 5, rr_depth=1, seed=k))) for k in range(1, 5)]
+    bd = [blocks(oracle.render(s, _params("bdpt", W, W, 512, 5, rr_depth=1, seed=k))) for k in range(1, 5)]
+    d = np.mean(pt, 0) - np.mean(bd, 0)
+    se = np.hypot(np.std(pt, 0, ddof=1), np.std(bd, 0, ddof=1)) / 2
+    assert np.mean(pt) > 0
+    assert np.all(np.abs(d) <= 4.5 * se + 1e-3 * np.mean(pt)), (d, se)
