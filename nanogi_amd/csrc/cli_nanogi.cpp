@@ -4,7 +4,7 @@
 // defaults and positional order (`nanogi [options] <renderer> <scene>
 // <result> <width> <height>`), usage on --help or no arguments (exit 0),
 // scene loaded with aspect W/H (:2069), render, SaveImage (:2097-2114).
-// Renderers other than pt/ptdirect/lt/ltdirect fail with "unsupported renderer".
+// Renderers other than pt/ptdirect/lt/ltdirect/bdpt fail with "unsupported renderer".
 // Additional options (not in the reference): --spp, --seed, --rr-depth,
 // --rr-prob, --precision, --engine, --paths-in-flight.
 #include <algorithm>
@@ -166,8 +166,8 @@ int run(int argc, char** argv) {
     ngi_render_params p;
     ngi_render_params_default(&p);
     const int r = ngi_renderer_from_string(vm["renderer"].c_str());
-    if (r != NGI_RENDERER_PT && r != NGI_RENDERER_PTDIRECT && r != NGI_RENDERER_LT && r != NGI_RENDERER_LTDIRECT) {
-        fprintf(stderr, "ERROR : unsupported renderer '%s' (this build provides pt, ptdirect, lt and ltdirect)\n", vm["renderer"].c_str());
+    if (r < NGI_RENDERER_PT || r > NGI_RENDERER_BDPT) {
+        fprintf(stderr, "ERROR : unsupported renderer '%s' (this build provides pt, ptdirect, lt, ltdirect and bdpt)\n", vm["renderer"].c_str());
         return EXIT_FAILURE;
     }
     p.renderer = r;
@@ -194,7 +194,7 @@ int run(int argc, char** argv) {
         fprintf(stderr, "ERROR : invalid image size\n");
         return EXIT_FAILURE;
     }
-    printf("nanogi (MI355X pt/ptdirect/lt/ltdirect)\n");
+    printf("nanogi (MI355X pt/ptdirect/lt/ltdirect/bdpt)\n");
     ngi_scene* scene = nullptr;
     printf("Loading scene\n");
     if (ngi_scene_load_yaml(vm["scene"].c_str(), (double)p.width / p.height, &scene) != NGI_OK) {

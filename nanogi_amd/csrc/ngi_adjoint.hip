@@ -216,7 +216,8 @@ static int adjoint_grid(size_t lds, unsigned& grid) {
 }
 
 int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* dLdI, double* gR_out, double* gLe_out, ngi_stats* stats) {
-    if (is_light_tracer(p.renderer)) return fail(NGI_ERR_UNSUPPORTED, "the adjoint differentiates pt and ptdirect");
+    if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT)
+        return fail(NGI_ERR_UNSUPPORTED, "the adjoint differentiates pt and ptdirect");
     Session S;
     ngi_render_params q = p;
     q.engine = NGI_ENGINE_PERSISTENT;

@@ -1,0 +1,439 @@
+// Bidirectional path tracing (bdpt) on persistent lanes: one path sample per
+// lane at a time, as Renderer::ProcessSample_BDPT (src/nanogi.cpp:1133-1189)
+// with Path (include/nanogi/bdpt.hpp).
+//
+// A sample draws a light subpath and an eye subpath (SampleSubpath,
+// bdpt.hpp:54-137) into per-lane vertex records in HBM, then evaluates every
+// strategy (s, t) with n = s + t vertices: a shadow ray between the subpath
+// ends (Connect, :139-191), the unweighted contribution
+// alphaL(s) * c(s,t) * alphaE(t) (:210-333), the power-heuristic weight over
+// all strategies that could produce the same path (:352-372) and the
+// Russian-roulette selection probability (:193-208); the result goes to the
+// pixel the camera end of the path projects to.
+//
+// The weight is computed differently from the reference's
+// EvaluatePowerHeuristicsMISWeightOpt, which recomputes every p_i(x) from
+// scratch (EvaluatePDF, :491-528): here p_i / p_s is walked outwards from s,
+// one vertex at a time, p_{i+1} / p_i = pL(x_i) / pE(x_i) with pL / pE the
+// area-measure pdfs of generating vertex x_i from the light / eye side; the
+// terms with c(i) = 0 are left out as in the reference (p_i = 0 there).  The
+// ratios are accumulated in FP64.
+//
+// Random numbers: the light subpath's step k draws stream 1 at vertex k
+// (step 0: light choice and position, as the light tracers), the eye
+// subpath's step k stream 0 at vertex k (step 1: the camera ray, stratified
+// in the pixel in per-pixel mode).
+#include <algorithm>
+
+#include "ngi_render.h"
+
+namespace ngi {
+
+#define NGI_BD_BLOCK 256
+// Vertices per subpath; the reference has no cap with -m -1 (with Russian
+// roulette at 0.5 a longer subpath has probability 2^-62).
+#define NGI_BDPT_MAX_VERTICES 64
+
+// One subpath vertex.  type: NGI_PRIM_L (light end), NGI_PRIM_E (camera),
+// else the surface's BSDF bits (0 on [L]-only surfaces, which absorb).
+struct BVert {
+    f3 p, gn, n;
+    f3 alpha;   // throughput of the subpath up to and including this vertex
+    int prim, type;
+    bool deg;   // pinhole or point light
+};
+
+struct BdptMem {
+    float4* v;       // 4 float4 per (side, vertex, lane), [side][vertex][lane]
+    uint32_t lanes;  // grid * block
+    int maxv;        // vertices per subpath
+};
+
+__device__ __forceinline__ float4* brec(const BdptMem& m, int side, int k, uint32_t lane) {
+    return m.v + ((size_t)(side * m.maxv + k) * m.lanes + lane) * 4;
+}
+
+__device__ __forceinline__ void bstore(const BdptMem& m, int side, int k, uint32_t lane, const BVert& v) {
+    float4* q = brec(m, side, k, lane);
+    q[0] = make_float4(v.p.x, v.p.y, v.p.z, __int_as_float(v.prim));
+    q[1] = make_float4(v.gn.x, v.gn.y, v.gn.z, __int_as_float(v.type | (v.deg ? 256 : 0)));
+    q[2] = make_float4(v.n.x, v.n.y, v.n.z, 0.0f);
+    q[3] = make_float4(v.alpha.x, v.alpha.y, v.alpha.z, 0.0f);
+}
+
+__device__ __forceinline__ BVert bload(const BdptMem& m, int side, int k, uint32_t lane) {
+    const float4* q = brec(m, side, k, lane);
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    BVert v;
+    v.p = f3(a.x, a.y, a.z);
+    v.prim = __float_as_int(a.w);
+    v.gn = f3(b.x, b.y, b.z);
+    const int tb = __float_as_int(b.w);
+    v.type = tb & 255;
+    v.deg = (tb & 256) != 0;
+    v.n = f3(c.x, c.y, c.z);
+    v.alpha = f3(d.x, d.y, d.z);
+    return v;
+}
+
+struct BdptCtx {
+    const DevScene* sc;
+    const DevPrim<float>* prims;
+    const DevLight<float>* lights;
+    Cam<float> cam;
+};
+
+__device__ __forceinline__ Surf<float> surf_of(const BVert& v) {
+    Surf<float> s;
+    s.p = v.p, s.gn = v.gn, s.n = v.n, s.degenerate = v.deg;
+    return s;
+}
+
+__device__ __forceinline__ const DevLight<float>& light_of(const BdptCtx& c, const BVert& v) { return c.lights[c.prims[v.prim].light]; }
+
+// EvaluateDirection(geom, type, wi, wo, LE / EL, forceDegenerated = false)
+// (rt.hpp:912-1148).  has_wi = false passes the reference's zero vector.
+template <bool LE>
+__device__ __forceinline__ f3 vert_eval(const BdptCtx& c, const BVert& v, bool has_wi, f3 wi, f3 wo) {
+    if (v.type == NGI_PRIM_L) {
+        const DevLight<float>& L = light_of(c, v);
+        const f3 Le(L.Le[0], L.Le[1], L.Le[2]);
+        if (L.type == NGI_LIGHT_AREA) return dot(v.n, wo) > 0.0f ? Le : f3(0.0f);
+        if (L.type == NGI_LIGHT_POINT) return Le;
+        return f3(0.0f);   // directional, not forced
+    }
+    if (v.type == NGI_PRIM_E) {
+        float rx, ry;
+        return f3(camera_importance(c.cam, wo, rx, ry));
+    }
+    return bsdf_eval_nee<float, LE>(c.prims[v.prim], v.type, surf_of(v), has_wi ? wi : f3(0.0f), wo);
+}
+
+// EvaluateDirectionPDF(geom, type, wi, wo, forceDegenerated = true) (rt.hpp:1150-1336).
+__device__ __forceinline__ float vert_pdf(const BdptCtx& c, const BVert& v, bool has_wi, f3 wi, f3 wo) {
+    if (v.type == NGI_PRIM_L) {
+        const DevLight<float>& L = light_of(c, v);
+        if (L.type == NGI_LIGHT_AREA) return dot(v.n, wo) > 0.0f ? (float)(1.0 / NGI_PI) : 0.0f;
+        if (L.type == NGI_LIGHT_POINT) return (float)(0.25 / NGI_PI);
+        return 1.0f;
+    }
+    if (v.type == NGI_PRIM_E) {
+        float rx, ry;
+        return camera_importance(c.cam, wo, rx, ry);
+    }
+    return bsdf_pdf<float>(c.prims[v.prim], v.type, surf_of(v), has_wi ? wi : f3(0.0f), wo);
+}
+
+// EvaluatePositionPDF(geom, true) * EvaluateEmitterPDF (rt.hpp:622-667, 2338-2352).
+__device__ __forceinline__ float pos_pdf(const BdptCtx& c, const BVert& v) {
+    if (v.type == NGI_PRIM_L) {
+        const DevLight<float>& L = light_of(c, v);
+        const float pa = L.type == NGI_LIGHT_POINT ? 1.0f : L.inv_area;
+        return pa / (float)c.sc->num_lights;
+    }
+    return 1.0f;   // pinhole, the only sensor
+}
+
+// EvaluatePosition(geom, false) (rt.hpp:564-620): zero for the degenerate
+// point light and pinhole, one otherwise.
+__device__ __forceinline__ float pos_eval_unforced(const BdptCtx& c, const BVert& v) {
+    if (v.type == NGI_PRIM_L) return light_of(c, v).type == NGI_LIGHT_POINT ? 0.0f : 1.0f;
+    return 0.0f;
+}
+
+__device__ __forceinline__ float gterm(const BVert& a, const BVert& b) {
+    f3 d = b.p - a.p;
+    const float l2 = dot(d, d);
+    d = d / sqrtf(l2);
+    float g = 1.0f;
+    if (!a.deg) g *= fabsf(dot(a.n, d));
+    if (!b.deg) g *= fabsf(dot(b.n, -d));
+    return g / l2;
+}
+
+__device__ __forceinline__ float rr_at(const RenderArgs& a, int step) {
+    return (a.rr_depth >= 0 && step >= a.rr_depth) ? a.rr_prob_f : 1.0f;
+}
+
+// The connected path x_0 .. x_{n-1} of strategy s (light end first).
+struct Conn {
+    const BdptMem* m;
+    uint32_t lane;
+    int s, n;
+    __device__ __forceinline__ BVert x(int k) const {
+        BVert v = k < s ? bload(*m, 0, k, lane) : bload(*m, 1, n - 1 - k, lane);
+        if (k == 0) v.type = NGI_PRIM_L;   // s = 0: the eye subpath's end on a light acts as the light (bdpt.hpp:150)
+        return v;
+    }
+};
+
+__device__ __forceinline__ f3 dirv(const BVert& from, const BVert& to) { return vnorm(to.p - from.p); }
+
+// c_i(x): EvaluateCst (bdpt.hpp:210-244) for the split after x_{i-1}.
+__device__ __forceinline__ f3 cst(const BdptCtx& c, const Conn& q, int i) {
+    const int n = q.n;
+    if (i == n) return f3(0.0f);   // t = 0: a pinhole is never hit (its EvaluatePosition(.., false) is zero)
+    if (i == 0) {
+        const BVert v0 = q.x(0), v1 = q.x(1);
+        return vert_eval<false>(c, v0, false, f3(0.0f), dirv(v0, v1)) * pos_eval_unforced(c, v0);
+    }
+    const BVert vL = q.x(i - 1), vE = q.x(i);
+    f3 wiL(0.0f), wiE(0.0f);
+    if (i - 2 >= 0) wiL = dirv(vL, q.x(i - 2));
+    if (i + 1 < n) wiE = dirv(vE, q.x(i + 1));
+    const f3 fsL = vert_eval<true>(c, vL, i - 2 >= 0, wiL, dirv(vL, vE));
+    if (is_zero(fsL)) return f3(0.0f);
+    const f3 fsE = vert_eval<false>(c, vE, i + 1 < n, wiE, dirv(vE, vL));
+    return fsL * fsE * gterm(vL, vE);
+}
+
+// Area-measure pdf of generating x_k from the light side (pL) or from the eye side (pE).
+__device__ __forceinline__ double pdf_light_side(const BdptCtx& c, const Conn& q, int k) {
+    if (k == 0) return (double)pos_pdf(c, q.x(0));
+    const BVert a = q.x(k - 1), b = q.x(k);
+    f3 wi(0.0f);
+    if (k - 2 >= 0) wi = dirv(a, q.x(k - 2));
+    return (double)vert_pdf(c, a, k - 2 >= 0, wi, dirv(a, b)) * (double)gterm(a, b);
+}
+
+__device__ __forceinline__ double pdf_eye_side(const BdptCtx& c, const Conn& q, int k) {
+    const int n = q.n;
+    if (k == n - 1) return (double)pos_pdf(c, q.x(n - 1));
+    const BVert a = q.x(k + 1), b = q.x(k);
+    f3 wi(0.0f);
+    if (k + 2 <= n - 1) wi = dirv(a, q.x(k + 2));
+    return (double)vert_pdf(c, a, k + 2 <= n - 1, wi, dirv(a, b)) * (double)gterm(a, b);
+}
+
+// Power-heuristic weight of strategy s: 1 / sum_i (p_i / p_s)^2 over the
+// strategies with c_i != 0.  Returns 0 when p_s itself vanishes.
+__device__ __forceinline__ double mis_weight(const BdptCtx& c, const Conn& q) {
+    const int s = q.s, n = q.n;
+    double inv = 1.0, r = 1.0;
+    for (int i = s - 1; i >= 0; --i) {   // p_i / p_{i+1} = pE(x_i) / pL(x_i)
+        const double pe = pdf_eye_side(c, q, i);
+        if (pe == 0.0) break;            // this and every smaller i have p_i = 0
+        r *= pe / pdf_light_side(c, q, i);
+        if (!is_zero(cst(c, q, i))) inv += r * r;
+    }
+    r = 1.0;
+    for (int i = s + 1; i < n; ++i) {    // p_i / p_{i-1} = pL(x_{i-1}) / pE(x_{i-1}); c_n = 0 for a pinhole
+        const double pl = pdf_light_side(c, q, i - 1);
+        if (pl == 0.0) break;
+        r *= pl / pdf_eye_side(c, q, i - 1);
+        if (!is_zero(cst(c, q, i))) inv += r * r;
+    }
+    const double w = 1.0 / inv;
+    return (w == w && inv < 1e300) ? w : 0.0;   // a strategy with p_s = 0 (never sampled) gets no weight
+}
+
+template <typename FilmT, int WIDTH, bool COUNT>
+__global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
+    extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
+    const DevScene& sc = a.sc;
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    BdptCtx c;
+    c.sc = &sc;
+    c.prims = prims_of<float>(sc);
+    c.lights = lights_of<float>(sc);
+    c.cam = load_cam<float>(sc.cam);
+    const uint32_t gl = blockIdx.x * NGI_BD_BLOCK + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint32_t* my_stk = stk + threadIdx.x;
+    const int maxv = mem.maxv;
+    for (;;) {
+        // One path id per active lane (one atomic per wavefront).
+        const unsigned long long act = __ballot(true);
+        const int leader = __ffsll((long long)act) - 1;
+        unsigned long long base = 0;
+        if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(act));
+        base = __shfl(base, leader);
+        const unsigned long long local = base + (unsigned long long)__popcll(act & ((1ull << lane) - 1ull));
+        if (local >= a.path_count) break;
+        uint32_t c0, c1;
+        int pixel;
+        path_coords(a, a.path_begin + local, c0, c1, pixel);
+        cnt[CNT_SAMPLES]++;
+        // ---- subpaths ----
+        int nsub[2] = {0, 0};
+        for (int side = 0; side < 2; ++side) {   // 0: light (LE), 1: eye (EL)
+            const uint32_t stream = side == 0 ? NGI_STREAM_NEE : NGI_STREAM_EXT;
+            BVert pv;
+            f3 ppv_p(0.0f);
+            if (side == 0) {
+                if (sc.num_lights == 0) continue;
+                const U4 r0 = rng4(c0, c1, 0u, stream, a.seed, a.stream_id);
+                const int li = pick_light(sc, u24f(r0.x));
+                const DevLight<float>& L = c.lights[li];
+                pv.prim = L.prim;
+                pv.type = NGI_PRIM_L;
+                if (L.type == NGI_LIGHT_AREA) {
+                    area_light_point(sc, L, u24f(r0.y), u24f(r0.z), pv.p, pv.gn);
+                    pv.n = pv.gn;
+                    pv.deg = false;
+                } else if (L.type == NGI_LIGHT_POINT) {
+                    pv.p = f3(L.pos[0], L.pos[1], L.pos[2]);
+                    pv.gn = pv.n = f3(0.0f);
+                    pv.deg = true;
+                } else {
+                    const f3 d(L.dir[0], L.dir[1], L.dir[2]);
+                    const Frame<float> fr = make_frame(d);
+                    float dx, dy;
+                    concentric_disk(u24f(r0.y), u24f(r0.z), dx, dy);
+                    pv.p = f3(L.pos[0], L.pos[1], L.pos[2]) - d * L.radius + (fr.s * (dx * L.radius) + fr.t * (dy * L.radius));
+                    pv.gn = pv.n = d;
+                    pv.deg = false;
+                }
+                pv.alpha = f3(1.0f / pos_pdf(c, pv));   // EvaluatePosition(.., true) = 1
+            } else {
+                pv.prim = sc.sensor_prim;
+                pv.type = NGI_PRIM_E;
+                pv.p = c.cam.eye;
+                pv.gn = pv.n = f3(0.0f);
+                pv.deg = true;
+                pv.alpha = f3(1.0f);
+            }
+            bstore(mem, side, 0, gl, pv);
+            int nv = 1;
+            for (int step = 1; step < maxv; ++step) {
+                const U4 r = rng4(c0, c1, (uint32_t)step, stream, a.seed, a.stream_id);
+                const float u0 = u24f(r.x), u1 = u24f(r.y), uc = u24f(r.z), urr = u24f(r.w);
+                const bool has_wi = step >= 2;
+                const f3 wi = has_wi ? vnorm(ppv_p - pv.p) : f3(0.0f);
+                f3 wo, wgt;
+                if (pv.type == NGI_PRIM_L) {
+                    const DevLight<float>& L = light_of(c, pv);
+                    const f3 Le(L.Le[0], L.Le[1], L.Le[2]);
+                    if (L.type == NGI_LIGHT_AREA) {
+                        wo = to_world(make_frame(pv.n), cosine_hemisphere(u0, u1));
+                        if (dot(pv.n, wo) <= 0.0f) break;
+                        wgt = Le * (float)NGI_PI;
+                    } else if (L.type == NGI_LIGHT_POINT) {
+                        wo = uniform_sphere(u0, u1);
+                        wgt = Le * (float)(4.0 * NGI_PI);
+                    } else {
+                        wo = f3(L.dir[0], L.dir[1], L.dir[2]);
+                        wgt = Le;
+                    }
+                } else if (pv.type == NGI_PRIM_E) {
+                    wo = camera_sample<float>(a, c.cam, u0, u1, pixel);
+                    wgt = f3(1.0f);   // pinhole fs / pdf = 1
+                } else {
+                    const bool ok = side == 0 ? bsdf_sample<float, true>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt)
+                                              : bsdf_sample<float, false>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt);
+                    if (!ok) break;
+                }
+                TraceRay er{pv.p.x, pv.p.y, pv.p.z, wo.x, wo.y, wo.z, NGI_EPS_F, FLT_MAX};
+                TraceHit h;
+                TraceCounters tc{0, 0};
+                const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, my_stk, NGI_BD_BLOCK);
+                cnt[CNT_EXT_RAYS]++;
+                if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+                if (!hit) break;
+                int hp;
+                const Surf<float> ns = hit_surface<float>(sc, pv.p, wo, h, hp);
+                BVert v;
+                v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
+                v.prim = hp;
+                v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                v.alpha = pv.alpha * wgt;
+                bstore(mem, side, step, gl, v);
+                nv = step + 1;
+                const float q = rr_at(a, step);
+                if (q < 1.0f && urr > q) break;   // the vertex is kept (bdpt.hpp:118-123)
+                ppv_p = pv.p;
+                pv = v;
+            }
+            nsub[side] = nv;
+        }
+        // ---- strategies ----
+        const int nL = nsub[0], nE = nsub[1];
+        for (int n = 2; n <= nL + nE; ++n) {
+            if (a.max_nv >= 0 && n > a.max_nv) break;
+            const int s_lo = n - nE > 0 ? n - nE : 0, s_hi = nL < n ? nL : n;
+            for (int s = s_lo; s <= s_hi; ++s) {
+                const int t = n - s;
+                if (t == 0) continue;   // a pinhole is never hit
+                Conn q{&mem, gl, s, n};
+                if (s == 0) {
+                    const BVert e = bload(mem, 1, t - 1, gl);
+                    if (!(c.prims[e.prim].type & NGI_PRIM_L)) continue;
+                } else {
+                    const BVert l = bload(mem, 0, s - 1, gl), e = bload(mem, 1, t - 1, gl);
+                    const f3 dv = e.p - l.p;
+                    const float len = sqrtf(dot(dv, dv));
+                    const f3 d = dv / len;
+                    TraceRay sr{l.p.x, l.p.y, l.p.z, d.x, d.y, d.z, NGI_EPS_F, len * (1.0f - NGI_EPS_F)};   // Scene::Visible
+                    TraceHit sh;
+                    TraceCounters tc{0, 0};
+                    const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, my_stk, NGI_BD_BLOCK);
+                    cnt[CNT_SHADOW_RAYS]++;
+                    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+                    if (occluded) continue;
+                }
+                const f3 cs = cst(c, q, s);
+                if (is_zero(cs)) continue;
+                const f3 aL = s > 0 ? bload(mem, 0, s - 1, gl).alpha : f3(1.0f);
+                const f3 aE = bload(mem, 1, t - 1, gl).alpha;
+                const f3 Cu = aL * cs * aE;
+                if (is_zero(Cu)) continue;
+                const double w = mis_weight(c, q);
+                if (w == 0.0) continue;
+                float sel = 1.0f;   // SelectionProb (bdpt.hpp:193-208)
+                for (int i = 1; i < s - 1; ++i) sel *= rr_at(a, i);
+                for (int i = t - 2; i >= 1; --i) sel *= rr_at(a, i);
+                const BVert cam_v = q.x(n - 1), prev = q.x(n - 2);
+                float rx = 0.0f, ry = 0.0f;
+                camera_importance(c.cam, dirv(cam_v, prev), rx, ry);
+                film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), Cu * (float)(w / (double)sel));
+            }
+        }
+    }
+    if (a.counters) {
+        for (int k = 0; k < CNT_N; ++k) {
+            if (!COUNT && (k == CNT_EXT_NODES || k == CNT_EXT_TRIS || k == CNT_SH_NODES || k == CNT_SH_TRIS)) continue;
+            unsigned long long x = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if ((threadIdx.x & 63) == 0 && x) atomicAdd(a.counters + k, x);
+        }
+    }
+}
+
+template <typename FilmT, int WIDTH, bool COUNT>
+static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, hipStream_t st) {
+    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_BD_BLOCK * 4 : 0;
+    auto kern = bdpt_kernel<FilmT, WIDTH, COUNT>;
+    int dev = 0, cus = 0, per_cu = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_BD_BLOCK, lds));
+    const uint64_t need_blocks = (a.path_count + NGI_BD_BLOCK - 1) / NGI_BD_BLOCK;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    BdptMem m;
+    m.lanes = grid * NGI_BD_BLOCK;
+    m.maxv = (a.max_nv < 0 || a.max_nv > NGI_BDPT_MAX_VERTICES) ? NGI_BDPT_MAX_VERTICES : a.max_nv;
+    const size_t bytes = (size_t)2 * m.maxv * m.lanes * 4 * sizeof(float4);
+    if (*mem_bytes < bytes) {
+        if (*mem) NGI_HIP_CHECK(hipFree(*mem));
+        *mem = nullptr;
+        *mem_bytes = 0;
+        if (hipMalloc(mem, bytes) != hipSuccess) {
+            *mem = nullptr;
+            return fail(NGI_ERR_OOM, "bdpt: cannot allocate %zu bytes of subpath records", bytes);
+        }
+        *mem_bytes = bytes;
+    }
+    m.v = reinterpret_cast<float4*>(*mem);
+    kern<<<grid, NGI_BD_BLOCK, lds, st>>>(a, next, m);
+    NGI_HIP_CHECK(hipGetLastError());
+    return NGI_OK;
+}
+
+int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
+                hipStream_t st) {
+    if (!film64) return width == 8 ? launch_bd<float, 8, false>(a, next, mem, mem_bytes, st) : launch_bd<float, 2, false>(a, next, mem, mem_bytes, st);
+    if (count) return width == 8 ? launch_bd<double, 8, true>(a, next, mem, mem_bytes, st) : launch_bd<double, 2, true>(a, next, mem, mem_bytes, st);
+    return width == 8 ? launch_bd<double, 8, false>(a, next, mem, mem_bytes, st) : launch_bd<double, 2, false>(a, next, mem, mem_bytes, st);
+}
+
+}  // namespace ngi

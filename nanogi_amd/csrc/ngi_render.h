@@ -23,6 +23,12 @@ int launch_persistent(const RenderArgs& a, unsigned long long* next, int precisi
 int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
 
 inline bool is_light_tracer(int renderer) { return renderer == NGI_RENDERER_LT || renderer == NGI_RENDERER_LTDIRECT; }
+// Bidirectional path tracer (ngi_bdpt.hip): FP32 shading; (re)allocates the
+// per-lane subpath records in *mem as needed.
+int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
+                hipStream_t st);
+// Renderers that always run on the persistent engine.
+inline bool persistent_only(int renderer) { return is_light_tracer(renderer) || renderer == NGI_RENDERER_BDPT; }
 
 struct Session {
     ngi_scene* scene = nullptr;
@@ -35,6 +41,8 @@ struct Session {
     void* m2 = nullptr;          // double[npix*3] sum of squares
     void* batch = nullptr;       // FilmT[npix*3] current batch film
     void* next_path = nullptr;   // unsigned long long path counter of the persistent engine
+    void* bdpt_mem = nullptr;    // bdpt subpath records
+    size_t bdpt_bytes = 0;
     uint64_t paths_done = 0;
     uint32_t samples_done = 0;
     int batches = 0;

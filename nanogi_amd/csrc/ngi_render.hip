@@ -181,14 +181,17 @@ int Session::zero(void* p, size_t n_elems, bool f64) {
 int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
     scene = sc;
     p = pp;
-    if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT && !is_light_tracer(p.renderer))
-        return fail(NGI_ERR_UNSUPPORTED, "renderer %d is not supported (pt, ptdirect, lt and ltdirect are)", p.renderer);
+    if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT && !is_light_tracer(p.renderer) &&
+        p.renderer != NGI_RENDERER_BDPT)
+        return fail(NGI_ERR_UNSUPPORTED, "renderer %d is not supported (pt, ptdirect, lt, ltdirect and bdpt are)", p.renderer);
     if (p.width <= 0 || p.height <= 0 || (int64_t)p.width * p.height > (1ll << 28)) return fail(NGI_ERR_INVALID_ARG, "bad image size");
     if (p.sampling != NGI_SAMPLING_PER_PIXEL && p.sampling != NGI_SAMPLING_REFERENCE_UNIFORM) return fail(NGI_ERR_INVALID_ARG, "bad sampling mode");
     if (p.sampling == NGI_SAMPLING_REFERENCE_UNIFORM && p.num_samples <= 0) return fail(NGI_ERR_INVALID_ARG, "num_samples must be positive");
     if (p.precision < 0 || p.precision > 2) return fail(NGI_ERR_INVALID_ARG, "bad precision");
     if (p.engine != NGI_ENGINE_MEGAKERNEL && p.engine != NGI_ENGINE_WAVEFRONT && p.engine != NGI_ENGINE_PERSISTENT)
         return fail(NGI_ERR_INVALID_ARG, "bad engine");
+    if (p.renderer == NGI_RENDERER_BDPT && p.precision == NGI_PRECISION_FP64_ALL)
+        return fail(NGI_ERR_UNSUPPORTED, "bdpt shades in FP32 (FP64 film with FP32_FP64ACC)");
     if (p.rr_depth >= 0 && !(p.rr_prob > 0.0 && p.rr_prob <= 1.0)) return fail(NGI_ERR_INVALID_ARG, "rr_prob must be in (0, 1]");
     if (p.max_num_vertices < -1) return fail(NGI_ERR_INVALID_ARG, "bad max_num_vertices");
     if (sc->sensor_prim < 0) return fail(NGI_ERR_INVALID_SCENE, "scene has no sensor (E) primitive");
@@ -220,8 +223,9 @@ int Session::reset() {
 
 void Session::destroy() {
     if (scene && scene->dev) (void)hipStreamSynchronize(stream());
-    for (void* q : {film, counters, m1, m2, batch, next_path}) if (q) (void)hipFree(q);
-    film = counters = m1 = m2 = batch = next_path = nullptr;
+    for (void* q : {film, counters, m1, m2, batch, next_path, bdpt_mem}) if (q) (void)hipFree(q);
+    film = counters = m1 = m2 = batch = next_path = bdpt_mem = nullptr;
+    bdpt_bytes = 0;
     release_engine();
     for (auto& e : pending) {
         (void)hipEventDestroy(e.a);
@@ -295,10 +299,10 @@ int Session::render_into(void* target, int spp) {
     uint64_t total;
     if (p.sampling == NGI_SAMPLING_PER_PIXEL) total = (uint64_t)npix * (uint64_t)spp;
     else total = (uint64_t)p.num_samples * (uint64_t)spp;
-    if (p.engine == NGI_ENGINE_WAVEFRONT && !is_light_tracer(p.renderer)) {
+    if (p.engine == NGI_ENGINE_WAVEFRONT && !persistent_only(p.renderer)) {
         st = render_wavefront(target, total);
-    } else if (p.engine == NGI_ENGINE_PERSISTENT || is_light_tracer(p.renderer)) {
-        // The light tracers always run on the persistent engine.
+    } else if (p.engine == NGI_ENGINE_PERSISTENT || persistent_only(p.renderer)) {
+        // The light tracers and bdpt always run on the persistent engine.
         if (!next_path) NGI_HIP_CHECK(hipMalloc(&next_path, sizeof(unsigned long long)));
         if ((st = zero(next_path, 1, true))) return st;
         RenderArgs a = args(target);
@@ -309,7 +313,9 @@ int Session::render_into(void* target, int spp) {
         if ((st = timed_begin(K_MEGA, (int64_t)total, tl))) return st;
         unsigned long long* next = reinterpret_cast<unsigned long long*>(next_path);
         const int width = scene->dev->view.bvh_width;
-        if (is_light_tracer(p.renderer))
+        if (p.renderer == NGI_RENDERER_BDPT)
+            st = launch_bdpt(a, next, &bdpt_mem, &bdpt_bytes, film64, width, p.collect_counters != 0, stream());
+        else if (is_light_tracer(p.renderer))
             st = launch_light(a, next, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, p.collect_counters != 0, stream());
         else
             st = launch_persistent(a, next, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, p.collect_counters != 0, stream());

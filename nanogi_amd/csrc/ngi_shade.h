@@ -289,6 +289,34 @@ __device__ __forceinline__ bool bsdf_sample(const DevPrim<R>& m, int type, const
     return false;
 }
 
+// EvaluateDirectionPDF (forceDegenerated = true) of a surface vertex, in
+// projected solid angle (rt.hpp:1213-1336): 1/pi for D, the Beckmann
+// half-vector pdf for G, the branch probability for S.
+template <typename R>
+__device__ __forceinline__ R bsdf_pdf(const DevPrim<R>& m, int type, const Surf<R>& s, V3<R> wi, V3<R> wo) {
+    const Frame<R> fr = make_frame(s.n);
+    const V3<R> li = to_local(fr, wi), lo = to_local(fr, wo);
+    if (type & NGI_PRIM_D) return (li.z <= R(0) || lo.z <= R(0)) ? R(0) : R(1.0 / NGI_PI);
+    if (type & NGI_PRIM_G) {
+        if (li.z <= R(0) || lo.z <= R(0)) return R(0);
+        const V3<R> H = vnorm(li + lo);
+        return beckmann_d(m, H) * H.z / (R(4) * dot(lo, H)) / lo.z;
+    }
+    if (type & NGI_PRIM_S) {
+        if (m.spec_type == NGI_SPEC_REFLECTION) return (li.z <= R(0) || lo.z <= R(0)) ? R(0) : R(1);
+        if (m.spec_type == NGI_SPEC_REFRACTION) return R(1);
+        R etaI = m.s_eta1, etaT = m.s_eta2;
+        if (li.z < R(0)) {
+            const R t = etaI;
+            etaI = etaT;
+            etaT = t;
+        }
+        const R Fr = fresnel_dielectric(li.z, etaI, etaT);
+        return li.z * lo.z >= R(0) ? Fr : R(1) - Fr;
+    }
+    return R(0);
+}
+
 // ---------------------------------------------------------------------------
 // Pinhole sensor (rt.hpp:733-740, 955-978, 1189-1212, 1348-1378).
 // ---------------------------------------------------------------------------

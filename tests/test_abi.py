@@ -71,7 +71,7 @@ def test_default_params_are_reference_defaults():
 
 def test_unsupported_renderers_fail_before_touching_the_device():
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
-    for r in ("bdpt", "ptmnee"):
+    for r in ("ptmnee",):
         with pytest.raises(ngi.NgiError) as e:
             ngi.render(s, renderer=r, width=8, height=8)
         assert e.value.status == ngi.ERR_UNSUPPORTED
@@ -101,13 +101,20 @@ def test_scene_without_sensor_is_rejected():
     assert e.value.status == ngi.ERR_INVALID_SCENE
 
 
-def test_adjoint_rejects_light_tracers():
+def test_adjoint_rejects_light_tracers_and_bdpt():
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
-    for r in ("lt", "ltdirect"):
+    for r in ("lt", "ltdirect", "bdpt"):
         p = ngi.default_params(renderer=r, width=4, height=4, sampling=ngi.SAMPLING_PER_PIXEL, spp=1)
         with pytest.raises(ngi.NgiError) as e:
             ngi.render_adjoint(s, p, np.zeros((4, 4, 3)))
         assert e.value.status == ngi.ERR_UNSUPPORTED
+
+
+def test_bdpt_rejects_fp64_shading():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.render(s, renderer="bdpt", width=4, height=4, precision=ngi.PRECISION_FP64_ALL)
+    assert e.value.status == ngi.ERR_UNSUPPORTED
 
 
 def test_no_gpu_reports_device_error():

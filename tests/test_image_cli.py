@@ -159,7 +159,7 @@ def test_cli_usage_exits_zero():
     assert _cli("--help").returncode == 0
 
 
-@pytest.mark.parametrize("args", [("-r", "bdpt", CORNELL, "x.hdr", "8", "8"), ("ptmnee", CORNELL), ("-i", CORNELL),
+@pytest.mark.parametrize("args", [("-r", "ptmnee", CORNELL, "x.hdr", "8", "8"), ("ptmnee", CORNELL), ("-i", CORNELL),
                                   ("pt", "/nonexistent.yml", "x.hdr", "8", "8"), ("--bogus", "1")])
 def test_cli_errors(args, tmp_path):
     r = _cli(*args)
