@@ -310,6 +310,43 @@ def rmse_study(a, scene, params, ngi):
     return out
 
 
+def bench_renderers(a):
+    """Every renderer on the 36-triangle Cornell box at 256x256, 64 spp, at
+    most 8 segments, no Russian roulette (the light tracers shoot the same
+    number of light paths): GPU samples/s on the persistent engine, and the
+    FP64 CPU oracle on a bounded sample of the same paths."""
+    import numpy as np
+
+    import nanogi_amd as ngi
+    import oracle
+    scene = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    scene.build(8)
+    W = H = 256
+    out = {"workload": "Cornell box (36 triangles), 256x256, 64 spp, max 8 segments, no Russian roulette"}
+    ol = oracle.LoadedScene(scene)
+    threads = min(a.cpu_threads, os.cpu_count() or 1)
+    for r in ("pt", "ptdirect", "lt", "ltdirect", "bdpt"):
+        p = ngi.default_params(renderer=r, width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=64, max_num_vertices=9,
+                               rr_depth=-1, seed=1, engine=ngi.ENGINE_PERSISTENT)
+        ngi.render(scene, p)   # warm-up (code load, allocations)
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            _, st = ngi.render(scene, p)
+        dt = (time.perf_counter() - t0) / reps
+        entry = {"gpu_samples_per_s": round(W * H * 64 / dt, 1), "gpu_rays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / dt, 1)}
+        if a.cpu_seconds > 0:
+            n = 65536
+            t0 = time.perf_counter()
+            ol.render_paths(p, 0, n, threads)
+            cdt = time.perf_counter() - t0
+            entry["cpu_samples_per_s"] = round(n / cdt, 1)
+            entry["cpu_threads"] = threads
+        out[r] = entry
+        log(f"{r}: {entry}")
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +371,7 @@ def parse():
                     help="scenes of the batched many-scene workload reported beside the main line (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering workload reported beside the main line (0 = skip)")
+    ap.add_argument("--renderers", type=int, default=1, help="time every renderer on the Cornell box (0 = skip)")
     ap.add_argument("--rmse-ref-spp", type=int, default=0,
                     help="render an FP64 reference at this spp and report each variant's RMSE against it (0 = skip)")
     ap.add_argument("--rmse-spp", type=int, default=1024, help="spp of each variant in the RMSE study")
@@ -519,6 +557,7 @@ def main():
     study = None
     if a.rmse_ref_spp > 0:
         study = rmse_study(a, scene, params, ngi)
+    renderers = bench_renderers(a) if a.renderers else None
     batch = None
     if a.batch_scenes > 0:
         batch = bench_batch(a)
@@ -547,6 +586,7 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "rmse_study": study,
+        "renderers": renderers,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -1,6 +1,6 @@
 /*
  * ngi.h - C ABI of the MI355X-native `pt` / `ptdirect` renderer (and the
- * `lt` / `ltdirect` light tracers).
+ * `lt` / `ltdirect` light tracers and the `bdpt` bidirectional path tracer).
  *
  * This is the drop-in boundary for nanogi's path- and light-tracing renderers.
  * Every entry point names the reference interface it replaces
@@ -37,7 +37,7 @@ typedef enum ngi_status {
     NGI_ERR_VERSION = 3,         /* version outside [3,5] (rt.hpp:1482-1486, 1535-1540) */
     NGI_ERR_INVALID_TYPE = 4,    /* empty type or [L,E] (rt.hpp:1600-1604) */
     NGI_ERR_INVALID_SCENE = 5,   /* area light w/o mesh (rt.hpp:1826-1830), sensor w/o UV (1921-1925), no mesh in file (1649-1652), missing D/G reflectance (1943-1946) */
-    NGI_ERR_UNSUPPORTED = 6,     /* renderer other than pt/ptdirect/lt/ltdirect, textures, area sensor */
+    NGI_ERR_UNSUPPORTED = 6,     /* ptmnee, textures, area sensor, unsupported precision/engine combinations */
     NGI_ERR_INVALID_ARG = 7,
     NGI_ERR_DEVICE = 8,          /* no GPU / HIP runtime error */
     NGI_ERR_OOM = 9,
@@ -200,7 +200,7 @@ int ngi_trace_device(ngi_scene* scene, const ngi_ray* d_rays, size_t n, ngi_hit*
 /* Rendering: RendererType + Renderer::Render (src/nanogi.cpp:51-71,   */
 /* 182-221), RenderProcess (225-440), ProcessSample_PT (446-607),      */
 /* ProcessSample_PTDirect (609-802), ProcessSample_LT (804-934),       */
-/* ProcessSample_LTDirect (936-1131).                                  */
+/* ProcessSample_LTDirect (936-1131), ProcessSample_BDPT (1133-1189).  */
 /* ------------------------------------------------------------------ */
 enum { NGI_RENDERER_PT = 0, NGI_RENDERER_PTDIRECT = 1, NGI_RENDERER_LT = 2,
        NGI_RENDERER_LTDIRECT = 3, NGI_RENDERER_BDPT = 4, NGI_RENDERER_PTMNEE = 5 };
@@ -217,7 +217,7 @@ enum { NGI_ENGINE_MEGAKERNEL = 0,   /* one path per lane */
        NGI_ENGINE_PERSISTENT = 2 }; /* persistent lanes with path regeneration */
 
 typedef struct ngi_render_params {
-    int renderer;            /* NGI_RENDERER_PT, _PTDIRECT, _LT or _LTDIRECT */
+    int renderer;            /* NGI_RENDERER_PT, _PTDIRECT, _LT, _LTDIRECT or _BDPT */
     int width, height;       /* -w/-h (src/nanogi.cpp:2006-2007) */
     int sampling;            /* NGI_SAMPLING_* */
     int64_t num_samples;     /* -n: total paths, REFERENCE_UNIFORM mode */
