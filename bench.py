@@ -366,11 +366,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-counters", action="store_true")
-    ap.add_argument("--paths-in-flight", type=int, default=0, help="wavefront queue capacity (0 = auto)")
+    ap.add_argument("--paths-in-flight", type=int, default=0,
+                    help="wavefront queue capacity or persistent lanes (0 = auto)")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes of the batched many-scene workload reported beside the main line (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering workload reported beside the main line (0 = skip)")
+    ap.add_argument("--autotune-paths", type=int, default=1 << 24,
+                    help="paths per candidate of the single-scene launch tuner on this workload (0 = skip)")
     ap.add_argument("--renderers", type=int, default=1, help="time every renderer on the Cornell box (0 = skip)")
     ap.add_argument("--rmse-ref-spp", type=int, default=0,
                     help="render an FP64 reference at this spp and report each variant's RMSE against it (0 = skip)")
@@ -525,6 +528,19 @@ def main():
         if scene.bvh_info()["width"] != a.bvh:
             scene.build(a.bvh)
 
+    # Single-scene launch tuner on the same workload (reported; the timed
+    # run above keeps the library defaults).
+    tune = None
+    if a.autotune_paths > 0:
+        names = {ngi.ENGINE_MEGAKERNEL: "megakernel", ngi.ENGINE_WAVEFRONT: "wavefront", ngi.ENGINE_PERSISTENT: "persistent"}
+        t1 = time.perf_counter()
+        tuned, trials = ngi.autotune(scene, params(), a.autotune_paths)
+        tune = {"tune_paths": a.autotune_paths, "tune_s": round(time.perf_counter() - t1, 3),
+                "chosen": {"engine": names[tuned.engine], "paths_in_flight": tuned.paths_in_flight},
+                "trials": [{"engine": names[t["engine"]], "paths_in_flight": t["paths_in_flight"],
+                            "ms": round(t["ms"], 3), "samples_per_s": round(t["samples_per_s"], 1)} for t in trials]}
+        log(f"autotune: {tune['chosen']} in {tune['tune_s']} s")
+
     # CPU baseline: the FP64 oracle on the host cores, bounded sample of paths.
     cpu = None
     if a.cpu_seconds > 0:
@@ -580,6 +596,7 @@ def main():
                           "time_to_1024spp_s": round(1024 * npix / value, 3)},
         "roofline": roof,
         "engines_samples_per_s": engines,
+        "autotune": tune,
         "cpu_baseline": cpu,
         "counters": counts,
         "bvh_build_s": round(info["build_ms"] / 1e3, 3),

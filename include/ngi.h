@@ -229,7 +229,8 @@ typedef struct ngi_render_params {
     uint32_t stream_id;      /* second key word (scene id in batched renders) */
     int precision;           /* NGI_PRECISION_* */
     int engine;              /* NGI_ENGINE_* */
-    int paths_in_flight;     /* wavefront capacity (0 = auto) */
+    int paths_in_flight;     /* concurrent paths: wavefront queue capacity, or lanes of the
+                              * persistent path-tracing kernel (0 = auto) */
     int block_size;          /* threads per block (0 = auto) */
     int collect_counters;    /* count node visits / triangle tests (slower) */
     int time_kernels;        /* record per-kernel HIP-event times */
@@ -280,6 +281,29 @@ int ngi_session_get_film(ngi_session* s, double* film_out, double* var_out);
 int ngi_session_get_stats(ngi_session* s, ngi_stats* out);
 int ngi_session_reset(ngi_session* s);
 void ngi_session_destroy(ngi_session* s);
+
+/* ------------------------------------------------------------------ */
+/* Launch tuning for one scene (no counterpart in the reference, whose */
+/* TBB loop has no engine to choose; it is the single-scene side of    */
+/* the launch autotuner).  Renders tune_paths paths (0 = 2^24) of the   */
+/* workload *p describes with each candidate -- persistent lanes at    */
+/* the full, 3/4 and 1/2 device fill; for pt/ptdirect also wavefront   */
+/* queues of 2^21, 2^22 and 2^23 paths (FP32 shading) and the          */
+/* megakernel -- as one warm-up and two timed renders (wall time, the  */
+/* better one counts), then writes the fastest into p->engine and      */
+/* p->paths_in_flight.  Every candidate draws the same random numbers, */
+/* so the choice changes speed, not the estimate.  trials (optional,   */
+/* max_trials entries) receives the timings in candidate order and     */
+/* *num_trials (optional) the number of candidates.                    */
+/* ------------------------------------------------------------------ */
+typedef struct ngi_tune_trial {
+    int engine;              /* NGI_ENGINE_* */
+    int paths_in_flight;     /* 0 = full device fill */
+    double ms;               /* wall time of one render of the sample */
+    double samples_per_s;
+} ngi_tune_trial;
+int ngi_autotune(ngi_scene* scene, ngi_render_params* p, int64_t tune_paths,
+                 ngi_tune_trial* trials, int max_trials, int* num_trials);
 
 /* ------------------------------------------------------------------ */
 /* Film output: SaveImage (include/nanogi/basic.hpp:506-672).  Writes  */

@@ -102,6 +102,10 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("engine", c_int), ("paths_in_flight", c_int), ("ms", c_double), ("samples_per_s", c_double)]
+
+
 class BatchOpts(ctypes.Structure):
     _fields_ = [("block_size", c_int), ("blocks_per_cu", c_int), ("use_graph", c_int), ("chunk_paths", c_int),
                 ("tune_paths", c_int64)]
@@ -116,7 +120,7 @@ EXPORTS = [
     "ngi_session_render_batch", "ngi_session_sync", "ngi_session_get_film", "ngi_session_get_stats",
     "ngi_session_reset", "ngi_session_destroy", "ngi_render_adjoint", "ngi_render_batch", "ngi_image_save",
     "ngi_batch_opts_default", "ngi_batch_create", "ngi_batch_render", "ngi_batch_sync", "ngi_batch_get_film",
-    "ngi_batch_get_times", "ngi_batch_get_opts", "ngi_batch_destroy",
+    "ngi_batch_get_times", "ngi_batch_get_opts", "ngi_batch_destroy", "ngi_autotune",
 ]
 
 
@@ -177,6 +181,7 @@ def lib():
         "ngi_batch_get_times": ([vp, vp, P(Stats)], c_int),
         "ngi_batch_get_opts": ([vp, P(BatchOpts), P(c_double)], c_int),
         "ngi_batch_destroy": ([vp], None),
+        "ngi_autotune": ([vp, P(RenderParams), ctypes.c_int64, P(TuneTrial), c_int, P(c_int)], c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -412,6 +417,21 @@ def render(scene, params=None, **kw):
     st = Stats()
     _check(lib().ngi_render(scene.handle, ctypes.byref(p), _ptr(film), ctypes.byref(st)))
     return film[:p.width * p.height * 3].reshape(p.height, p.width, 3), st.as_dict()
+
+
+def autotune(scene, params, tune_paths=0):
+    """Time every engine / concurrency candidate on a sample of `tune_paths`
+    paths (0 = 2^24) of the workload `params` describes (ngi_autotune).
+
+    Returns (tuned, trials): a copy of `params` with the fastest engine and
+    paths_in_flight, and one dict per candidate (engine, paths_in_flight, ms,
+    samples_per_s) in candidate order."""
+    q = RenderParams()
+    ctypes.pointer(q)[0] = params
+    trials = (TuneTrial * 16)()
+    n = c_int(0)
+    _check(lib().ngi_autotune(scene.handle, ctypes.byref(q), int(tune_paths), trials, 16, ctypes.byref(n)))
+    return q, [{name: getattr(trials[i], name) for name, _ in TuneTrial._fields_} for i in range(min(n.value, 16))]
 
 
 def render_batch(scenes, params):

@@ -6,7 +6,7 @@
 // scene loaded with aspect W/H (:2069), render, SaveImage (:2097-2114).
 // Renderers other than pt/ptdirect/lt/ltdirect/bdpt fail with "unsupported renderer".
 // Additional options (not in the reference): --spp, --seed, --rr-depth,
-// --rr-prob, --precision, --engine, --paths-in-flight.
+// --rr-prob, --precision, --engine, --paths-in-flight, --autotune.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -50,7 +50,8 @@ const Opt kOpts[] = {
     {"rr-prob", 0, "0.5", "Russian roulette continuation probability", false},
     {"precision", 0, "fp32", "fp32 (FP64 film) | fp32all | fp64", false},
     {"engine", 0, "persistent", "persistent | wavefront | megakernel", false},
-    {"paths-in-flight", 0, "0", "Wavefront queue capacity (0: auto)", false},
+    {"paths-in-flight", 0, "0", "Concurrent paths: wavefront queue capacity or persistent lanes (0: auto)", false},
+    {"autotune", 0, nullptr, "Time the engines on a sample of this many paths and render with the fastest", false},
 };
 const char* kPositional[] = {"renderer", "scene", "result", "width", "height"};
 
@@ -200,6 +201,19 @@ int run(int argc, char** argv) {
     if (ngi_scene_load_yaml(vm["scene"].c_str(), (double)p.width / p.height, &scene) != NGI_OK) {
         fprintf(stderr, "ERROR : %s\n", ngi_last_error());
         return EXIT_FAILURE;
+    }
+    if (vm.count("autotune")) {
+        ngi_tune_trial trials[16];
+        int n = 0;
+        if (ngi_autotune(scene, &p, atoll(vm["autotune"].c_str()), trials, 16, &n) != NGI_OK) {
+            fprintf(stderr, "ERROR : %s\n", ngi_last_error());
+            ngi_scene_destroy(scene);
+            return EXIT_FAILURE;
+        }
+        const char* names[] = {"megakernel", "wavefront", "persistent"};
+        for (int i = 0; i < std::min(n, 16); ++i)
+            printf("Autotune: %-10s paths in flight %8d: %.3f ms\n", names[trials[i].engine], trials[i].paths_in_flight, trials[i].ms);
+        printf("Autotune: using %s, paths in flight %d\n", names[p.engine], p.paths_in_flight);
     }
     printf("Rendering\n");
     const double render_time = atof(vm["render-time"].c_str());

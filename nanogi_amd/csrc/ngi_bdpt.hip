@@ -400,15 +400,12 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsign
 }
 
 template <typename FilmT, int WIDTH, bool COUNT>
-static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, hipStream_t st) {
+static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_BD_BLOCK * 4 : 0;
     auto kern = bdpt_kernel<FilmT, WIDTH, COUNT>;
-    int dev = 0, cus = 0, per_cu = 0;
-    NGI_HIP_CHECK(hipGetDevice(&dev));
-    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_BD_BLOCK, lds));
-    const uint64_t need_blocks = (a.path_count + NGI_BD_BLOCK - 1) / NGI_BD_BLOCK;
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    unsigned grid = 0;
+    int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_BD_BLOCK, lds, a.path_count, ln, &grid);
+    if (s || !grid) return s;
     BdptMem m;
     m.lanes = grid * NGI_BD_BLOCK;
     m.maxv = (a.max_nv < 0 || a.max_nv > NGI_BDPT_MAX_VERTICES) ? NGI_BDPT_MAX_VERTICES : a.max_nv;
@@ -430,10 +427,12 @@ static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, 
 }
 
 int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
-                hipStream_t st) {
-    if (!film64) return width == 8 ? launch_bd<float, 8, false>(a, next, mem, mem_bytes, st) : launch_bd<float, 2, false>(a, next, mem, mem_bytes, st);
-    if (count) return width == 8 ? launch_bd<double, 8, true>(a, next, mem, mem_bytes, st) : launch_bd<double, 2, true>(a, next, mem, mem_bytes, st);
-    return width == 8 ? launch_bd<double, 8, false>(a, next, mem, mem_bytes, st) : launch_bd<double, 2, false>(a, next, mem, mem_bytes, st);
+                hipStream_t st, Lanes ln) {
+    if (!film64)
+        return width == 8 ? launch_bd<float, 8, false>(a, next, mem, mem_bytes, ln, st) : launch_bd<float, 2, false>(a, next, mem, mem_bytes, ln, st);
+    if (count)
+        return width == 8 ? launch_bd<double, 8, true>(a, next, mem, mem_bytes, ln, st) : launch_bd<double, 2, true>(a, next, mem, mem_bytes, ln, st);
+    return width == 8 ? launch_bd<double, 8, false>(a, next, mem, mem_bytes, ln, st) : launch_bd<double, 2, false>(a, next, mem, mem_bytes, ln, st);
 }
 
 }  // namespace ngi

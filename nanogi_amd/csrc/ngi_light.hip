@@ -55,31 +55,29 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsig
 }
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-static int launch_lt(const RenderArgs& a, unsigned long long* next, hipStream_t st) {
+static int launch_lt(const RenderArgs& a, unsigned long long* next, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4 : 0;
     auto kern = light_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
-    int dev = 0, cus = 0, per_cu = 0;
-    NGI_HIP_CHECK(hipGetDevice(&dev));
-    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_LT_BLOCK, lds));
-    const uint64_t need_blocks = (a.path_count + NGI_LT_BLOCK - 1) / NGI_LT_BLOCK;
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    unsigned grid = 0;
+    int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_LT_BLOCK, lds, a.path_count, ln, &grid);
+    if (s || !grid) return s;
     kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
 template <typename R, typename FilmT, bool COUNT>
-static int launch_lt3(const RenderArgs& a, unsigned long long* next, bool direct, int width, hipStream_t st) {
-    if (direct) return width == 8 ? launch_lt<R, FilmT, true, 8, COUNT>(a, next, st) : launch_lt<R, FilmT, true, 2, COUNT>(a, next, st);
-    return width == 8 ? launch_lt<R, FilmT, false, 8, COUNT>(a, next, st) : launch_lt<R, FilmT, false, 2, COUNT>(a, next, st);
+static int launch_lt3(const RenderArgs& a, unsigned long long* next, bool direct, int width, Lanes ln, hipStream_t st) {
+    if (direct) return width == 8 ? launch_lt<R, FilmT, true, 8, COUNT>(a, next, ln, st) : launch_lt<R, FilmT, true, 2, COUNT>(a, next, ln, st);
+    return width == 8 ? launch_lt<R, FilmT, false, 8, COUNT>(a, next, ln, st) : launch_lt<R, FilmT, false, 2, COUNT>(a, next, ln, st);
 }
 
-int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st) {
-    if (precision == NGI_PRECISION_FP64_ALL) return launch_lt3<double, double, false>(a, next, direct, width, st);
-    if (precision == NGI_PRECISION_FP32_ALL) return launch_lt3<float, float, false>(a, next, direct, width, st);
-    if (count) return launch_lt3<float, double, true>(a, next, direct, width, st);
-    return launch_lt3<float, double, false>(a, next, direct, width, st);
+int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
+                 Lanes ln) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_lt3<double, double, false>(a, next, direct, width, ln, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_lt3<float, float, false>(a, next, direct, width, ln, st);
+    if (count) return launch_lt3<float, double, true>(a, next, direct, width, ln, st);
+    return launch_lt3<float, double, false>(a, next, direct, width, ln, st);
 }
 
 }  // namespace ngi

@@ -1,6 +1,7 @@
 // Render session shared by the megakernel and wavefront engines.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "ngi_path.h"
@@ -17,16 +18,43 @@ struct TimedLaunch {
 
 struct WavefrontState;   // ngi_wavefront.hip
 
+// Concurrency of a persistent launch.  want > 0 caps the paths in flight
+// (rounded up to whole blocks); 0 keeps every block the device can hold
+// resident; want < 0 launches nothing and only reports that full size.
+struct Lanes {
+    int64_t want = 0;
+    int64_t* full = nullptr;   // optional: full size in lanes
+};
+
+// Grid of a persistent kernel under `ln`; *grid = 0 for a query (want < 0).
+inline int persistent_grid(const void* kern, int block, size_t lds, uint64_t paths, Lanes ln, unsigned* grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    NGI_HIP_CHECK(hipGetDevice(&dev));
+    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds));
+    const uint64_t full = (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, per_cu);
+    if (ln.full) *ln.full = (int64_t)(full * (uint64_t)block);
+    *grid = 0;
+    if (ln.want < 0) return NGI_OK;
+    uint64_t blocks = full;
+    if (ln.want > 0) blocks = std::min<uint64_t>(full, std::max<uint64_t>(1, ((uint64_t)ln.want + block - 1) / block));
+    const uint64_t need = (paths + block - 1) / block;
+    *grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need, blocks));
+    return NGI_OK;
+}
+
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st);
-int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
+int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
+                      Lanes ln = Lanes());
 // Light tracers (ngi_light.hip): lt (direct = false) and ltdirect.
-int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st);
+int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
+                 Lanes ln = Lanes());
 
 inline bool is_light_tracer(int renderer) { return renderer == NGI_RENDERER_LT || renderer == NGI_RENDERER_LTDIRECT; }
 // Bidirectional path tracer (ngi_bdpt.hip): FP32 shading; (re)allocates the
 // per-lane subpath records in *mem as needed.
 int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
-                hipStream_t st);
+                hipStream_t st, Lanes ln = Lanes());
 // Renderers that always run on the persistent engine.
 inline bool persistent_only(int renderer) { return is_light_tracer(renderer) || renderer == NGI_RENDERER_BDPT; }
 
@@ -57,6 +85,8 @@ struct Session {
     int zero(void* p, size_t n_elems, bool f64);
     RenderArgs args(void* target_film) const;
     int render_into(void* target, int spp);
+    // Launch the persistent kernel of this renderer under `ln`.
+    int launch_pk_engine(const RenderArgs& a, Lanes ln);
     int render(int spp);
     int render_batch(int spp);
     int sync();

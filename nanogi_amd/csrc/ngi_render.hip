@@ -114,31 +114,29 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
 }
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-static int launch_pk(const RenderArgs& a, unsigned long long* next, hipStream_t st) {
+static int launch_pk(const RenderArgs& a, unsigned long long* next, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
     auto kern = persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
-    int dev = 0, cus = 0, per_cu = 0;
-    NGI_HIP_CHECK(hipGetDevice(&dev));
-    NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NGI_MK_BLOCK, lds));
-    const uint64_t need_blocks = (a.path_count + NGI_MK_BLOCK - 1) / NGI_MK_BLOCK;
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(need_blocks, (uint64_t)std::max(1, cus) * std::max(1, per_cu)));
+    unsigned grid = 0;
+    int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_MK_BLOCK, lds, a.path_count, ln, &grid);
+    if (s || !grid) return s;
     kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
 template <typename R, typename FilmT, bool COUNT>
-static int launch_pk3(const RenderArgs& a, unsigned long long* next, bool direct, int width, hipStream_t st) {
-    if (direct) return width == 8 ? launch_pk<R, FilmT, true, 8, COUNT>(a, next, st) : launch_pk<R, FilmT, true, 2, COUNT>(a, next, st);
-    return width == 8 ? launch_pk<R, FilmT, false, 8, COUNT>(a, next, st) : launch_pk<R, FilmT, false, 2, COUNT>(a, next, st);
+static int launch_pk3(const RenderArgs& a, unsigned long long* next, bool direct, int width, Lanes ln, hipStream_t st) {
+    if (direct) return width == 8 ? launch_pk<R, FilmT, true, 8, COUNT>(a, next, ln, st) : launch_pk<R, FilmT, true, 2, COUNT>(a, next, ln, st);
+    return width == 8 ? launch_pk<R, FilmT, false, 8, COUNT>(a, next, ln, st) : launch_pk<R, FilmT, false, 2, COUNT>(a, next, ln, st);
 }
 
-int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st) {
-    if (precision == NGI_PRECISION_FP64_ALL) return launch_pk3<double, double, false>(a, next, direct, width, st);
-    if (precision == NGI_PRECISION_FP32_ALL) return launch_pk3<float, float, false>(a, next, direct, width, st);
-    if (count) return launch_pk3<float, double, true>(a, next, direct, width, st);
-    return launch_pk3<float, double, false>(a, next, direct, width, st);
+int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
+                      Lanes ln) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_pk3<double, double, false>(a, next, direct, width, ln, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_pk3<float, float, false>(a, next, direct, width, ln, st);
+    if (count) return launch_pk3<float, double, true>(a, next, direct, width, ln, st);
+    return launch_pk3<float, double, false>(a, next, direct, width, ln, st);
 }
 
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st) {
@@ -311,14 +309,9 @@ int Session::render_into(void* target, int spp) {
         a.path_count = total;
         TimedLaunch tl;
         if ((st = timed_begin(K_MEGA, (int64_t)total, tl))) return st;
-        unsigned long long* next = reinterpret_cast<unsigned long long*>(next_path);
-        const int width = scene->dev->view.bvh_width;
-        if (p.renderer == NGI_RENDERER_BDPT)
-            st = launch_bdpt(a, next, &bdpt_mem, &bdpt_bytes, film64, width, p.collect_counters != 0, stream());
-        else if (is_light_tracer(p.renderer))
-            st = launch_light(a, next, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, p.collect_counters != 0, stream());
-        else
-            st = launch_persistent(a, next, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, p.collect_counters != 0, stream());
+        Lanes ln;
+        ln.want = std::max(0, p.paths_in_flight);
+        st = launch_pk_engine(a, ln);
         if (!st) st = timed_end(tl);
     } else {
         // Launches of at most 2^22 paths (whole pixels rows of samples).
@@ -346,6 +339,16 @@ int Session::render_into(void* target, int spp) {
     samples_done += (uint32_t)spp;
     stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return NGI_OK;
+}
+
+int Session::launch_pk_engine(const RenderArgs& a, Lanes ln) {
+    unsigned long long* next = reinterpret_cast<unsigned long long*>(next_path);
+    const int width = scene->dev->view.bvh_width;
+    const bool count = p.collect_counters != 0;
+    if (p.renderer == NGI_RENDERER_BDPT) return launch_bdpt(a, next, &bdpt_mem, &bdpt_bytes, film64, width, count, stream(), ln);
+    if (is_light_tracer(p.renderer))
+        return launch_light(a, next, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, count, stream(), ln);
+    return launch_persistent(a, next, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, count, stream(), ln);
 }
 
 int Session::render(int spp) { return render_into(film, spp); }
@@ -482,6 +485,82 @@ void ngi_session_destroy(ngi_session* s) {
     if (!s) return;
     s->destroy();
     delete s;
+}
+
+int ngi_autotune(ngi_scene* scene, ngi_render_params* p, int64_t tune_paths, ngi_tune_trial* trials, int max_trials, int* num_trials) {
+    clear_error();
+    if (!scene || !p) return fail(NGI_ERR_INVALID_ARG, "null argument");
+    if (tune_paths < 0 || max_trials < 0 || (max_trials > 0 && !trials)) return fail(NGI_ERR_INVALID_ARG, "bad tuning arguments");
+    if (num_trials) *num_trials = 0;
+    if (tune_paths == 0) tune_paths = (int64_t)1 << 24;
+    ngi_render_params q = *p;
+    q.collect_counters = 0;
+    q.time_kernels = 0;
+    q.engine = NGI_ENGINE_PERSISTENT;
+    q.paths_in_flight = 0;
+    int passes = 1;
+    if (q.sampling == NGI_SAMPLING_PER_PIXEL) {
+        const int64_t npix = std::max<int64_t>(1, (int64_t)q.width * q.height);
+        passes = (int)std::min<int64_t>(std::max<int64_t>(1, tune_paths / npix), 1 << 16);
+    } else {
+        q.num_samples = tune_paths;
+    }
+    Session s;
+    int st = s.init(scene, q);
+    int64_t full = 0;
+    if (!st) {
+        RenderArgs a = s.args(s.film);
+        a.path_count = 1;
+        Lanes query;
+        query.want = -1;
+        query.full = &full;
+        st = s.launch_pk_engine(a, query);
+    }
+    struct Cand {
+        int engine;
+        int64_t lanes;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    for (int64_t l : {(int64_t)0, full * 3 / 4, full / 2}) cands.push_back({NGI_ENGINE_PERSISTENT, l, 0.0});
+    if (!persistent_only(q.renderer)) {
+        if (q.precision != NGI_PRECISION_FP64_ALL)
+            for (int64_t c : {(int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) cands.push_back({NGI_ENGINE_WAVEFRONT, c, 0.0});
+        cands.push_back({NGI_ENGINE_MEGAKERNEL, 0, 0.0});
+    }
+    // Each candidate: one untimed warm-up render, then the best of two.
+    for (size_t i = 0; i < cands.size() && !st; ++i) {
+        s.release_engine();
+        s.p.engine = cands[i].engine;
+        s.p.paths_in_flight = (int)cands[i].lanes;
+        double best = 0.0;
+        for (int rep = 0; rep < 3 && !st; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            st = s.render(passes);
+            if (!st) st = s.sync();
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rep == 1 || (rep > 1 && ms < best)) best = ms;
+        }
+        cands[i].ms = best;
+    }
+    const std::string err = st ? std::string(ngi_last_error()) : std::string();
+    s.destroy();
+    if (st) return fail(st, "autotune: %s", err.c_str());
+    size_t bi = 0;
+    for (size_t i = 1; i < cands.size(); ++i)
+        if (cands[i].ms < cands[bi].ms) bi = i;
+    p->engine = cands[bi].engine;
+    p->paths_in_flight = (int)cands[bi].lanes;
+    const double paths = (double)passes * (double)(q.sampling == NGI_SAMPLING_PER_PIXEL ? (int64_t)q.width * q.height : q.num_samples);
+    const int n = (int)std::min<size_t>(cands.size(), (size_t)max_trials);
+    for (int i = 0; i < n; ++i) {
+        trials[i].engine = cands[i].engine;
+        trials[i].paths_in_flight = (int)cands[i].lanes;
+        trials[i].ms = cands[i].ms;
+        trials[i].samples_per_s = cands[i].ms > 0 ? paths / (cands[i].ms * 1e-3) : 0.0;
+    }
+    if (num_trials) *num_trials = (int)cands.size();
+    return NGI_OK;
 }
 
 int ngi_render(ngi_scene* scene, const ngi_render_params* p, double* film, ngi_stats* stats) {

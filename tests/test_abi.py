@@ -34,7 +34,8 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof from a C compiler against the ctypes mirror."""
     structs = {"ngi_primitive": ngi.Primitive, "ngi_scene_desc": ngi.SceneDesc, "ngi_bvh_opts": ngi.BvhOpts,
                "ngi_bvh_info": ngi.BvhInfo, "ngi_ray": ngi.Ray, "ngi_hit": ngi.Hit,
-               "ngi_render_params": ngi.RenderParams, "ngi_stats": ngi.Stats, "ngi_batch_opts": ngi.BatchOpts}
+               "ngi_render_params": ngi.RenderParams, "ngi_stats": ngi.Stats, "ngi_batch_opts": ngi.BatchOpts,
+               "ngi_tune_trial": ngi.TuneTrial}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -115,6 +116,21 @@ def test_bdpt_rejects_fp64_shading():
     with pytest.raises(ngi.NgiError) as e:
         ngi.render(s, renderer="bdpt", width=4, height=4, precision=ngi.PRECISION_FP64_ALL)
     assert e.value.status == ngi.ERR_UNSUPPORTED
+
+
+def test_autotune_arguments_are_checked_before_the_device():
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    ok = dict(width=8, height=8, spp=1, sampling=ngi.SAMPLING_PER_PIXEL)
+    with pytest.raises(ngi.NgiError) as e:
+        ngi.autotune(s, ngi.default_params(**ok), tune_paths=-1)
+    assert e.value.status == ngi.ERR_INVALID_ARG
+    for kw, status in [(dict(width=0), ngi.ERR_INVALID_ARG), (dict(renderer="ptmnee"), ngi.ERR_UNSUPPORTED),
+                       (dict(renderer="bdpt", precision=ngi.PRECISION_FP64_ALL), ngi.ERR_UNSUPPORTED)]:
+        p = ngi.default_params(**dict(ok, **kw))
+        with pytest.raises(ngi.NgiError) as e:
+            ngi.autotune(s, p)
+        assert e.value.status == status, kw
+        assert "autotune" in e.value.message
 
 
 def test_no_gpu_reports_device_error():

@@ -181,6 +181,17 @@ def test_cli_renders_and_matches_library(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_autotune(tmp_path):
+    out = tmp_path / "img.pfm"
+    r = _cli("-r", "ptdirect", "-i", CORNELL, "-o", str(out), "-w", "40", "-h", "30", "--spp", "16", "-m", "6",
+             "--seed", "3", "--autotune", "100000")
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("Autotune:") == 8 and "# of samples: 19200" in r.stdout
+    img = _read_pfm(out)
+    assert img.shape == (30, 40, 3) and np.isfinite(img).all() and img.mean() > 0
+
+
+@pytest.mark.gpu
 def test_cli_reference_sampling_and_progress_images(tmp_path):
     out = tmp_path / "img.hdr"
     fmt = str(tmp_path / "prog" / "{{count}}.pfm")
