@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsign
 }
 
 template <typename FilmT, int WIDTH, bool COUNT>
-static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, Lanes ln, hipStream_t st) {
+static int launch_bd(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_BD_BLOCK * 4 : 0;
     auto kern = bdpt_kernel<FilmT, WIDTH, COUNT>;
     unsigned grid = 0;
@@ -410,29 +410,17 @@ static int launch_bd(const RenderArgs& a, unsigned long long* next, void** mem, 
     m.lanes = grid * NGI_BD_BLOCK;
     m.maxv = (a.max_nv < 0 || a.max_nv > NGI_BDPT_MAX_VERTICES) ? NGI_BDPT_MAX_VERTICES : a.max_nv;
     const size_t bytes = (size_t)2 * m.maxv * m.lanes * 4 * sizeof(float4);
-    if (*mem_bytes < bytes) {
-        if (*mem) NGI_HIP_CHECK(hipFree(*mem));
-        *mem = nullptr;
-        *mem_bytes = 0;
-        if (hipMalloc(mem, bytes) != hipSuccess) {
-            *mem = nullptr;
-            return fail(NGI_ERR_OOM, "bdpt: cannot allocate %zu bytes of subpath records", bytes);
-        }
-        *mem_bytes = bytes;
-    }
-    m.v = reinterpret_cast<float4*>(*mem);
+    if ((s = mem.reserve(bytes))) return s;
+    m.v = reinterpret_cast<float4*>(*mem.ptr);
     kern<<<grid, NGI_BD_BLOCK, lds, st>>>(a, next, m);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
-int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
-                hipStream_t st, Lanes ln) {
-    if (!film64)
-        return width == 8 ? launch_bd<float, 8, false>(a, next, mem, mem_bytes, ln, st) : launch_bd<float, 2, false>(a, next, mem, mem_bytes, ln, st);
-    if (count)
-        return width == 8 ? launch_bd<double, 8, true>(a, next, mem, mem_bytes, ln, st) : launch_bd<double, 2, true>(a, next, mem, mem_bytes, ln, st);
-    return width == 8 ? launch_bd<double, 8, false>(a, next, mem, mem_bytes, ln, st) : launch_bd<double, 2, false>(a, next, mem, mem_bytes, ln, st);
+int launch_bdpt(const RenderArgs& a, unsigned long long* next, LaneMem mem, bool film64, int width, bool count, hipStream_t st, Lanes ln) {
+    if (!film64) return width == 8 ? launch_bd<float, 8, false>(a, next, mem, ln, st) : launch_bd<float, 2, false>(a, next, mem, ln, st);
+    if (count) return width == 8 ? launch_bd<double, 8, true>(a, next, mem, ln, st) : launch_bd<double, 2, true>(a, next, mem, ln, st);
+    return width == 8 ? launch_bd<double, 8, false>(a, next, mem, ln, st) : launch_bd<double, 2, false>(a, next, mem, ln, st);
 }
 
 }  // namespace ngi

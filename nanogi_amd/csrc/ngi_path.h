@@ -60,6 +60,48 @@ __device__ __forceinline__ void film_add(void* film, int pixel, V3<R> v) {
     if (v.z != R(0)) unsafeAtomicAdd(f + 2, (FilmT)v.z);
 }
 
+// Deferred shadow rays of the persistent engine.  Only about half of the
+// NEE samples of a vertex need a shadow ray, so tracing them where they are
+// made runs the any-hit traversal with half of the wavefront idle.  Instead
+// each wavefront appends its shadow rays (with their contribution and pixel)
+// to its own ring of NGI_SHQ records in HBM and traces them 64 at a time.
+// The film receives the same contributions; only the order of the FP64
+// additions changes.  Record k of slot s of wavefront w is
+// rec[(k * waves + w) * NGI_SHQ + s]: (o, tmax), (d, C.r), (C.g, C.b, pixel, -).
+#define NGI_SHQ 128
+struct ShadowQueue {
+    float4* rec;
+    uint32_t waves;
+};
+__device__ __forceinline__ float4* shq_at(const ShadowQueue& q, int k, uint32_t wave, uint32_t slot) {
+    return q.rec + ((size_t)k * q.waves + wave) * NGI_SHQ + (slot & (NGI_SHQ - 1u));
+}
+// Appends this lane's shadow ray at `slot`.
+__device__ __forceinline__ void shq_push(const ShadowQueue& q, uint32_t wave, uint32_t slot, const TraceRay& r, V3<float> C, int pixel) {
+    *shq_at(q, 0, wave, slot) = make_float4(r.ox, r.oy, r.oz, r.tmax);
+    *shq_at(q, 1, wave, slot) = make_float4(r.dx, r.dy, r.dz, C.x);
+    *shq_at(q, 2, wave, slot) = make_float4(C.y, C.z, __int_as_float(pixel), 0.0f);
+}
+// Lanes 0..n-1 trace the n oldest shadow rays from `head` and splat the
+// unoccluded ones (Scene::Visible, rt.hpp:2251-2261; src/nanogi.cpp:700-706).
+template <typename FilmT, int WIDTH, bool COUNT>
+__device__ __forceinline__ void shq_trace(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t head, uint32_t n,
+                                          uint32_t* stk, int stride, uint32_t* cnt) {
+    if (__lane_id() >= n) return;
+    const uint32_t slot = head + __lane_id();
+    const float4 r0 = *shq_at(q, 0, wave, slot), r1 = *shq_at(q, 1, wave, slot), r2 = *shq_at(q, 2, wave, slot);
+    TraceRay sr;
+    sr.ox = r0.x, sr.oy = r0.y, sr.oz = r0.z, sr.tmax = r0.w;
+    sr.dx = r1.x, sr.dy = r1.y, sr.dz = r1.z;
+    sr.tmin = NGI_EPS_F;
+    TraceHit sh;
+    TraceCounters tc{0, 0};
+    const bool occluded = trace<WIDTH, true, COUNT>(a.sc, sr, sh, tc, stk, stride);
+    cnt[CNT_SHADOW_RAYS]++;
+    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+    if (!occluded) film_add<FilmT>(a.film, __float_as_int(r2.z), V3<float>(r1.w, r2.x, r2.y));
+}
+
 template <int WIDTH, bool ANY, bool COUNT>
 __device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, TraceHit& h, TraceCounters& c, uint32_t* stk, int stride) {
     if (WIDTH == 8) return trace_bvh8<ANY, COUNT>(sc, r, h, c, stk, stride);
@@ -202,10 +244,13 @@ __device__ __forceinline__ void path_start(const RenderArgs& a, uint64_t id, con
 
 // One iteration of the vertex loop (src/nanogi.cpp:483-606 / 645-801):
 // NEE (ptdirect), direction sample, extension ray, emission (pt), Russian
-// roulette.  Returns false when the path ends.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+// roulette.  Returns false when the path ends.  With QUEUE the NEE shadow ray
+// is appended to the wavefront's shadow queue at slot qtail + (rank among the
+// lanes appending) and *pushed is set, instead of being traced here.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
 __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
-                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt) {
+                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
+                                            uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
     const DevScene& sc = a.sc;
     if (ps.nv >= max_nv) return false;
     if (DIRECT) {
@@ -213,7 +258,12 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
         TraceRay sr;
         int target;
         const V3<R> C = nee_contribution<R>(sc, cam, prims, ps.type, ps.prim, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
-        if (!is_zero(C)) {
+        if (QUEUE && !is_zero(C)) {
+            const unsigned long long m = __ballot(1);
+            shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr,
+                     V3<float>((float)C.x, (float)C.y, (float)C.z), target);
+            *pushed = true;
+        } else if (!is_zero(C)) {
             TraceHit sh;
             TraceCounters tc{0, 0};
             const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, stk, stride);

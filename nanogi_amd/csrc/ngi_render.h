@@ -43,9 +43,28 @@ inline int persistent_grid(const void* kern, int block, size_t lds, uint64_t pat
     return NGI_OK;
 }
 
+// Device scratch owned by a session and grown on demand (per-lane bdpt
+// subpath records, per-wavefront shadow queues).
+struct LaneMem {
+    void** ptr;
+    size_t* bytes;
+    int reserve(size_t n) const {
+        if (*bytes >= n) return NGI_OK;
+        if (*ptr) NGI_HIP_CHECK(hipFree(*ptr));
+        *ptr = nullptr;
+        *bytes = 0;
+        if (hipMalloc(ptr, n) != hipSuccess) {
+            *ptr = nullptr;
+            return fail(NGI_ERR_OOM, "cannot allocate %zu bytes of per-lane records", n);
+        }
+        *bytes = n;
+        return NGI_OK;
+    }
+};
+
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st);
-int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
-                      Lanes ln = Lanes());
+int launch_persistent(const RenderArgs& a, unsigned long long* next, LaneMem mem, int precision, bool direct, int width, bool count,
+                      hipStream_t st, Lanes ln = Lanes());
 // Light tracers (ngi_light.hip): lt (direct = false) and ltdirect.
 int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
                  Lanes ln = Lanes());
@@ -53,8 +72,8 @@ int launch_light(const RenderArgs& a, unsigned long long* next, int precision, b
 inline bool is_light_tracer(int renderer) { return renderer == NGI_RENDERER_LT || renderer == NGI_RENDERER_LTDIRECT; }
 // Bidirectional path tracer (ngi_bdpt.hip): FP32 shading; (re)allocates the
 // per-lane subpath records in *mem as needed.
-int launch_bdpt(const RenderArgs& a, unsigned long long* next, void** mem, size_t* mem_bytes, bool film64, int width, bool count,
-                hipStream_t st, Lanes ln = Lanes());
+int launch_bdpt(const RenderArgs& a, unsigned long long* next, LaneMem mem, bool film64, int width, bool count, hipStream_t st,
+                Lanes ln = Lanes());
 // Renderers that always run on the persistent engine.
 inline bool persistent_only(int renderer) { return is_light_tracer(renderer) || renderer == NGI_RENDERER_BDPT; }
 
@@ -69,8 +88,8 @@ struct Session {
     void* m2 = nullptr;          // double[npix*3] sum of squares
     void* batch = nullptr;       // FilmT[npix*3] current batch film
     void* next_path = nullptr;   // unsigned long long path counter of the persistent engine
-    void* bdpt_mem = nullptr;    // bdpt subpath records
-    size_t bdpt_bytes = 0;
+    void* lane_mem = nullptr;    // bdpt subpath records or ptdirect shadow queues
+    size_t lane_bytes = 0;
     uint64_t paths_done = 0;
     uint32_t samples_done = 0;
     int batches = 0;

@@ -8,6 +8,7 @@
 // W*H/N scaling (src/nanogi.cpp:434-437) is applied on read-back.
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "ngi_path.h"
@@ -60,14 +61,19 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
 // Persistent lanes with path regeneration: a lane whose path ended takes the
 // next path id (one atomic per wavefront and iteration), so every iteration
 // traces with all lanes busy regardless of how uneven path lengths are.
+// With ptdirect in FP32 the NEE shadow rays go through the wavefront's
+// shadow queue (q, a ring per wavefront) and are traced 64 at a time.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next) {
+__global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     const Cam<R> cam = load_cam<R>(a.sc.cam);
     const DevPrim<R>* prims = prims_of<R>(a.sc);
     const int max_nv = clamp_vertices(a.max_nv);
     const uint32_t lane = __lane_id();
+    const uint32_t wave = blockIdx.x * (NGI_MK_BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform: oldest slot, queued rays (< NGI_SHQ)
     PathState<R> ps;
     bool active = false, exhausted = false;
     for (;;) {
@@ -89,8 +95,22 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(
                 }
             }
         }
-        if (__ballot(active) == 0) break;
-        if (active) active = path_vertex<R, FilmT, DIRECT, WIDTH, COUNT>(a, cam, prims, ps, stk + threadIdx.x, NGI_MK_BLOCK, max_nv, cnt);
+        const bool any = __ballot(active) != 0;
+        if (QUEUE) {
+            // Full rounds of 64; once every path has ended, the remainder.
+            while (qcount >= 64u || (!any && qcount > 0u)) {
+                const uint32_t n = qcount < 64u ? qcount : 64u;
+                shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, qhead, n, stk + threadIdx.x, NGI_MK_BLOCK, cnt);
+                qhead += n;
+                qcount -= n;
+            }
+        }
+        if (!any) break;
+        bool pushed = false;
+        if (active)
+            active = path_vertex<R, FilmT, DIRECT, WIDTH, COUNT, QUEUE>(a, cam, prims, ps, stk + threadIdx.x, NGI_MK_BLOCK, max_nv, cnt,
+                                                                        &q, wave, qhead + qcount, &pushed);
+        if (QUEUE) qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
     }
     flush_counters(a.counters, cnt, COUNT);
 }
@@ -114,29 +134,37 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
 }
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-static int launch_pk(const RenderArgs& a, unsigned long long* next, Lanes ln, hipStream_t st) {
+static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
     auto kern = persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_MK_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;
-    kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next);
+    ShadowQueue q;
+    q.waves = grid * (NGI_MK_BLOCK / 64);
+    q.rec = nullptr;
+    if (DIRECT && std::is_same<R, float>::value) {
+        if ((s = mem.reserve((size_t)3 * q.waves * NGI_SHQ * sizeof(float4)))) return s;
+        q.rec = reinterpret_cast<float4*>(*mem.ptr);
+    }
+    kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next, q);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
 template <typename R, typename FilmT, bool COUNT>
-static int launch_pk3(const RenderArgs& a, unsigned long long* next, bool direct, int width, Lanes ln, hipStream_t st) {
-    if (direct) return width == 8 ? launch_pk<R, FilmT, true, 8, COUNT>(a, next, ln, st) : launch_pk<R, FilmT, true, 2, COUNT>(a, next, ln, st);
-    return width == 8 ? launch_pk<R, FilmT, false, 8, COUNT>(a, next, ln, st) : launch_pk<R, FilmT, false, 2, COUNT>(a, next, ln, st);
+static int launch_pk3(const RenderArgs& a, unsigned long long* next, LaneMem mem, bool direct, int width, Lanes ln, hipStream_t st) {
+    if (direct)
+        return width == 8 ? launch_pk<R, FilmT, true, 8, COUNT>(a, next, mem, ln, st) : launch_pk<R, FilmT, true, 2, COUNT>(a, next, mem, ln, st);
+    return width == 8 ? launch_pk<R, FilmT, false, 8, COUNT>(a, next, mem, ln, st) : launch_pk<R, FilmT, false, 2, COUNT>(a, next, mem, ln, st);
 }
 
-int launch_persistent(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
-                      Lanes ln) {
-    if (precision == NGI_PRECISION_FP64_ALL) return launch_pk3<double, double, false>(a, next, direct, width, ln, st);
-    if (precision == NGI_PRECISION_FP32_ALL) return launch_pk3<float, float, false>(a, next, direct, width, ln, st);
-    if (count) return launch_pk3<float, double, true>(a, next, direct, width, ln, st);
-    return launch_pk3<float, double, false>(a, next, direct, width, ln, st);
+int launch_persistent(const RenderArgs& a, unsigned long long* next, LaneMem mem, int precision, bool direct, int width, bool count,
+                      hipStream_t st, Lanes ln) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_pk3<double, double, false>(a, next, mem, direct, width, ln, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_pk3<float, float, false>(a, next, mem, direct, width, ln, st);
+    if (count) return launch_pk3<float, double, true>(a, next, mem, direct, width, ln, st);
+    return launch_pk3<float, double, false>(a, next, mem, direct, width, ln, st);
 }
 
 void launch_megakernel(const RenderArgs& a, int precision, bool direct, int width, bool count, hipStream_t st) {
@@ -221,9 +249,9 @@ int Session::reset() {
 
 void Session::destroy() {
     if (scene && scene->dev) (void)hipStreamSynchronize(stream());
-    for (void* q : {film, counters, m1, m2, batch, next_path, bdpt_mem}) if (q) (void)hipFree(q);
-    film = counters = m1 = m2 = batch = next_path = bdpt_mem = nullptr;
-    bdpt_bytes = 0;
+    for (void* q : {film, counters, m1, m2, batch, next_path, lane_mem}) if (q) (void)hipFree(q);
+    film = counters = m1 = m2 = batch = next_path = lane_mem = nullptr;
+    lane_bytes = 0;
     release_engine();
     for (auto& e : pending) {
         (void)hipEventDestroy(e.a);
@@ -345,10 +373,11 @@ int Session::launch_pk_engine(const RenderArgs& a, Lanes ln) {
     unsigned long long* next = reinterpret_cast<unsigned long long*>(next_path);
     const int width = scene->dev->view.bvh_width;
     const bool count = p.collect_counters != 0;
-    if (p.renderer == NGI_RENDERER_BDPT) return launch_bdpt(a, next, &bdpt_mem, &bdpt_bytes, film64, width, count, stream(), ln);
+    LaneMem mem{&lane_mem, &lane_bytes};
+    if (p.renderer == NGI_RENDERER_BDPT) return launch_bdpt(a, next, mem, film64, width, count, stream(), ln);
     if (is_light_tracer(p.renderer))
         return launch_light(a, next, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, count, stream(), ln);
-    return launch_persistent(a, next, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, count, stream(), ln);
+    return launch_persistent(a, next, mem, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, count, stream(), ln);
 }
 
 int Session::render(int spp) { return render_into(film, spp); }
