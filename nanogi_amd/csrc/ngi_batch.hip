@@ -42,6 +42,7 @@ struct BatchDev {
     unsigned long long num_chunks;
     uint32_t chunk_paths;                          // paths per chunk (the last chunk of a scene may be shorter)
     int num_scenes;
+    ShadowQueue q;                                 // ptdirect: per-wave shadow-ray rings
 };
 
 // misc words: wall clock at start (atomic min), chunk counter, ray counters.
@@ -74,12 +75,18 @@ __device__ __forceinline__ int owner(const unsigned long long* __restrict__ p, i
 // works on the same scene and the scene's arguments are wave-uniform.
 // Completed paths are counted per wave and added to the scene's counter
 // when the wave leaves the scene; the addition that completes the count
-// stamps the scene's completion time.
+// stamps the scene's completion time.  With ptdirect the NEE shadow rays go
+// through the wave's shadow queue (traced 64 at a time, like the
+// single-scene persistent engine); the queue only ever holds rays of the
+// wave's current scene, since it is drained before the wave counts its
+// finished paths and leaves the scene.
 template <bool DIRECT, int WIDTH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev b) {
     extern __shared__ uint32_t stk[];   // max stack_entries over the scenes x BLOCK (wide BVH only)
     if (threadIdx.x == 0) atomicMin(b.misc + MISC_T_START, (unsigned long long)wall_clock64());
     const uint32_t lane = __lane_id();
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform shadow queue state (< NGI_SHQ)
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     PathState<float> ps;
@@ -127,7 +134,17 @@ __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev 
                 next += took < avail ? took : avail;
             }
         }
-        if (__ballot(active) == 0) {
+        const bool any = __ballot(active) != 0;
+        if (DIRECT) {
+            // Full rounds of 64; all of them before the wave leaves the scene.
+            while (qcount >= 64u || (!any && qcount > 0u)) {
+                const uint32_t nq = qcount < 64u ? qcount : 64u;
+                shq_trace<double, WIDTH, false>(a, b.q, wave, qhead, nq, stk + threadIdx.x, BLOCK, cnt);
+                qhead += nq;
+                qcount -= nq;
+            }
+        }
+        if (!any) {
             // The wave has no path in flight: count what it finished on this
             // scene, then move to the waiting chunk, the next chunk, or stop.
             if (si >= 0 && ended) {
@@ -147,9 +164,12 @@ __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev 
             break;
         }
         const bool was = active;
+        bool pushed = false;
         if (active)
-            active = path_vertex<float, double, DIRECT, WIDTH, false>(a, load_cam<float>(a.sc.cam), prims_of<float>(a.sc), ps,
-                                                                      stk + threadIdx.x, BLOCK, clamp_vertices(a.max_nv), cnt);
+            active = path_vertex<float, double, DIRECT, WIDTH, false, DIRECT>(a, load_cam<float>(a.sc.cam), prims_of<float>(a.sc), ps,
+                                                                              stk + threadIdx.x, BLOCK, clamp_vertices(a.max_nv), cnt,
+                                                                              &b.q, wave, qhead + qcount, &pushed);
+        if (DIRECT) qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
         ended += (unsigned long long)__popcll(__ballot(was && !active));
     }
     wave_sum_add(b.misc + MISC_EXT, cnt[CNT_EXT_RAYS]);
@@ -203,6 +223,8 @@ struct Batch {
     unsigned long long *d_off = nullptr, *d_done = nullptr, *d_tdone = nullptr, *d_misc = nullptr, *d_tune_off = nullptr;
     unsigned long long *d_chunk = nullptr, *d_tune_chunk = nullptr;   // chunk prefix sums (n + 1)
     unsigned long long num_chunks = 0;
+    float4* shq = nullptr;          // ptdirect shadow queues, sized for the most resident waves of any block size
+    size_t shq_waves = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t frame = 0;
@@ -227,6 +249,8 @@ struct Batch {
         b.num_chunks = nchunks;
         b.chunk_paths = c;
         b.num_scenes = n();
+        b.q.rec = shq;
+        b.q.waves = 0;
         return b;
     }
     size_t lds(int block) const { return width == 8 ? (size_t)stack * block * 4 : 0; }
@@ -248,7 +272,11 @@ struct Batch {
         BatchKernel k = pick_kernel(direct, width, block);
         if (!k) return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", block);
         const unsigned grid = (unsigned)std::max(1, num_cus * per_cu);
-        k<<<grid, block, lds(block), stream>>>(dev(off, chunk, nchunks, c));
+        BatchDev d = dev(off, chunk, nchunks, c);
+        d.q.waves = grid * (unsigned)(block / 64);
+        if (direct && d.q.waves > shq_waves)
+            return fail(NGI_ERR_INVALID_ARG, "batch launch of %u waves exceeds the %zu shadow queues", d.q.waves, shq_waves);
+        k<<<grid, block, lds(block), stream>>>(d);
         NGI_HIP_CHECK(hipGetLastError());
         return NGI_OK;
     }
@@ -389,6 +417,11 @@ int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, c
         (st = dev_alloc(&d_done, count)) || (st = dev_alloc(&d_tdone, count)) ||
         (st = dev_alloc(&d_misc, MISC_N)))
         return st;
+    if (direct) {
+        // One shadow ring per wave the device can hold at any block size.
+        for (int block : {64, 128, 256}) shq_waves = std::max(shq_waves, (size_t)num_cus * max_per_cu(block) * (size_t)(block / 64));
+        if ((st = dev_alloc(&shq, 3 * shq_waves * NGI_SHQ))) return st;
+    }
     for (int i = 0; i < count; ++i) args[i].film = film + film_off[i];
     NGI_HIP_CHECK(hipMemcpyAsync(d_args, args.data(), count * sizeof(RenderArgs), hipMemcpyHostToDevice, stream));
     NGI_HIP_CHECK(hipMemcpyAsync(d_off, offset.data(), (count + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
@@ -542,8 +575,9 @@ void Batch::destroy() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* q : {(void*)film, (void*)d_args, (void*)d_off, (void*)d_tune_off, (void*)d_chunk, (void*)d_tune_chunk, (void*)d_done,
-                    (void*)d_tdone, (void*)d_misc})
+                    (void*)d_tdone, (void*)d_misc, (void*)shq})
         if (q) (void)hipFree(q);
+    shq = nullptr, shq_waves = 0;
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
