@@ -78,9 +78,12 @@ __device__ __forceinline__ bool light_start(const RenderArgs& a, uint64_t id, Li
 }
 
 // One iteration of the light-path loop.  Returns false when the path ends.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+// With QUEUE the sensor connection's shadow ray is appended to the
+// wavefront's shadow queue (as in path_vertex) instead of being traced here.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
 __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, LightState<R>& ls,
-                                             uint32_t* stk, int stride, int max_nv, uint32_t* cnt) {
+                                             uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
+                                             uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
     const DevScene& sc = a.sc;
     if (ls.nv >= max_nv) return false;
     const DevLight<R>& L = lights_of<R>(sc)[ls.light];
@@ -117,12 +120,19 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
                     sr.dx = (float)ppE.x, sr.dy = (float)ppE.y, sr.dz = (float)ppE.z;
                     sr.tmin = NGI_EPS_F;
                     sr.tmax = (float)len * (1.0f - NGI_EPS_F);   // Scene::Visible (rt.hpp:2251-2261)
-                    TraceHit sh;
-                    TraceCounters tc{0, 0};
-                    const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, stk, stride);
-                    cnt[CNT_SHADOW_RAYS]++;
-                    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
-                    if (!occluded) film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), C);
+                    if (QUEUE) {
+                        const unsigned long long m = __ballot(1);
+                        shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr,
+                                 V3<float>((float)C.x, (float)C.y, (float)C.z), pixel_index(rx, ry, a.w, a.h));
+                        *pushed = true;
+                    } else {
+                        TraceHit sh;
+                        TraceCounters tc{0, 0};
+                        const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, stk, stride);
+                        cnt[CNT_SHADOW_RAYS]++;
+                        if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+                        if (!occluded) film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), C);
+                    }
                 }
             }
         }

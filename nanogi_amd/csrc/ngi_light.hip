@@ -2,6 +2,7 @@
 // next light-path id when their path ends (one atomic per wavefront), as the
 // path tracers' persistent kernel does.  The vertex loop is in ngi_light.h.
 #include <algorithm>
+#include <type_traits>
 
 #include "ngi_light.h"
 #include "ngi_render.h"
@@ -10,14 +11,19 @@ namespace ngi {
 
 #define NGI_LT_BLOCK 256
 
+// ltdirect in FP32 sends its sensor connections through the wavefront's
+// shadow queue and traces them 64 at a time.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsigned long long* next) {
+__global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
     const Cam<R> cam = load_cam<R>(a.sc.cam);
     const DevPrim<R>* prims = prims_of<R>(a.sc);
     const int max_nv = clamp_vertices(a.max_nv);
     const uint32_t lane = __lane_id();
+    const uint32_t wave = blockIdx.x * (NGI_LT_BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform shadow queue state (< NGI_SHQ)
     LightState<R> ls;
     bool active = false, exhausted = false;
     for (;;) {
@@ -38,11 +44,25 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsig
                 }
             }
         }
-        if (__ballot(active) == 0) {
+        const bool any = __ballot(active) != 0;
+        if (QUEUE) {
+            // Full rounds of 64; the remainder once every path has ended.
+            while (qcount >= 64u || (!any && exhausted && qcount > 0u)) {
+                const uint32_t n = qcount < 64u ? qcount : 64u;
+                shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, qhead, n, stk + threadIdx.x, NGI_LT_BLOCK, cnt);
+                qhead += n;
+                qcount -= n;
+            }
+        }
+        if (!any) {
             if (exhausted) break;
             continue;
         }
-        if (active) active = light_vertex<R, FilmT, DIRECT, WIDTH, COUNT>(a, cam, prims, ls, stk + threadIdx.x, NGI_LT_BLOCK, max_nv, cnt);
+        bool pushed = false;
+        if (active)
+            active = light_vertex<R, FilmT, DIRECT, WIDTH, COUNT, QUEUE>(a, cam, prims, ls, stk + threadIdx.x, NGI_LT_BLOCK, max_nv, cnt, &q,
+                                                                         wave, qhead + qcount, &pushed);
+        if (QUEUE) qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
     }
     if (a.counters) {
         for (int k = 0; k < CNT_N; ++k) {
@@ -55,29 +75,37 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsig
 }
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-static int launch_lt(const RenderArgs& a, unsigned long long* next, Lanes ln, hipStream_t st) {
+static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4 : 0;
     auto kern = light_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_LT_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;
-    kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next);
+    ShadowQueue q;
+    q.waves = grid * (NGI_LT_BLOCK / 64);
+    q.rec = nullptr;
+    if (DIRECT && std::is_same<R, float>::value) {
+        if ((s = mem.reserve((size_t)3 * q.waves * NGI_SHQ * sizeof(float4)))) return s;
+        q.rec = reinterpret_cast<float4*>(*mem.ptr);
+    }
+    kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next, q);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
 template <typename R, typename FilmT, bool COUNT>
-static int launch_lt3(const RenderArgs& a, unsigned long long* next, bool direct, int width, Lanes ln, hipStream_t st) {
-    if (direct) return width == 8 ? launch_lt<R, FilmT, true, 8, COUNT>(a, next, ln, st) : launch_lt<R, FilmT, true, 2, COUNT>(a, next, ln, st);
-    return width == 8 ? launch_lt<R, FilmT, false, 8, COUNT>(a, next, ln, st) : launch_lt<R, FilmT, false, 2, COUNT>(a, next, ln, st);
+static int launch_lt3(const RenderArgs& a, unsigned long long* next, LaneMem mem, bool direct, int width, Lanes ln, hipStream_t st) {
+    if (direct)
+        return width == 8 ? launch_lt<R, FilmT, true, 8, COUNT>(a, next, mem, ln, st) : launch_lt<R, FilmT, true, 2, COUNT>(a, next, mem, ln, st);
+    return width == 8 ? launch_lt<R, FilmT, false, 8, COUNT>(a, next, mem, ln, st) : launch_lt<R, FilmT, false, 2, COUNT>(a, next, mem, ln, st);
 }
 
-int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
-                 Lanes ln) {
-    if (precision == NGI_PRECISION_FP64_ALL) return launch_lt3<double, double, false>(a, next, direct, width, ln, st);
-    if (precision == NGI_PRECISION_FP32_ALL) return launch_lt3<float, float, false>(a, next, direct, width, ln, st);
-    if (count) return launch_lt3<float, double, true>(a, next, direct, width, ln, st);
-    return launch_lt3<float, double, false>(a, next, direct, width, ln, st);
+int launch_light(const RenderArgs& a, unsigned long long* next, LaneMem mem, int precision, bool direct, int width, bool count,
+                 hipStream_t st, Lanes ln) {
+    if (precision == NGI_PRECISION_FP64_ALL) return launch_lt3<double, double, false>(a, next, mem, direct, width, ln, st);
+    if (precision == NGI_PRECISION_FP32_ALL) return launch_lt3<float, float, false>(a, next, mem, direct, width, ln, st);
+    if (count) return launch_lt3<float, double, true>(a, next, mem, direct, width, ln, st);
+    return launch_lt3<float, double, false>(a, next, mem, direct, width, ln, st);
 }
 
 }  // namespace ngi

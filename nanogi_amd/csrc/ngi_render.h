@@ -66,8 +66,8 @@ void launch_megakernel(const RenderArgs& a, int precision, bool direct, int widt
 int launch_persistent(const RenderArgs& a, unsigned long long* next, LaneMem mem, int precision, bool direct, int width, bool count,
                       hipStream_t st, Lanes ln = Lanes());
 // Light tracers (ngi_light.hip): lt (direct = false) and ltdirect.
-int launch_light(const RenderArgs& a, unsigned long long* next, int precision, bool direct, int width, bool count, hipStream_t st,
-                 Lanes ln = Lanes());
+int launch_light(const RenderArgs& a, unsigned long long* next, LaneMem mem, int precision, bool direct, int width, bool count,
+                 hipStream_t st, Lanes ln = Lanes());
 
 inline bool is_light_tracer(int renderer) { return renderer == NGI_RENDERER_LT || renderer == NGI_RENDERER_LTDIRECT; }
 // Bidirectional path tracer (ngi_bdpt.hip): FP32 shading; (re)allocates the

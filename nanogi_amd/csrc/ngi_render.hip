@@ -376,7 +376,7 @@ int Session::launch_pk_engine(const RenderArgs& a, Lanes ln) {
     LaneMem mem{&lane_mem, &lane_bytes};
     if (p.renderer == NGI_RENDERER_BDPT) return launch_bdpt(a, next, mem, film64, width, count, stream(), ln);
     if (is_light_tracer(p.renderer))
-        return launch_light(a, next, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, count, stream(), ln);
+        return launch_light(a, next, mem, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, count, stream(), ln);
     return launch_persistent(a, next, mem, p.precision, p.renderer == NGI_RENDERER_PTDIRECT, width, count, stream(), ln);
 }
 
