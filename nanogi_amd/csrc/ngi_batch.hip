@@ -135,15 +135,8 @@ __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev 
             }
         }
         const bool any = __ballot(active) != 0;
-        if (DIRECT) {
-            // Full rounds of 64; all of them before the wave leaves the scene.
-            while (qcount >= 64u || (!any && qcount > 0u)) {
-                const uint32_t nq = qcount < 64u ? qcount : 64u;
-                shq_trace<double, WIDTH, false>(a, b.q, wave, qhead, nq, stk + threadIdx.x, BLOCK, cnt);
-                qhead += nq;
-                qcount -= nq;
-            }
-        }
+        // All queued rays are traced before the wave leaves the scene.
+        if (DIRECT) shq_flush<double, WIDTH, false>(a, b.q, wave, qhead, qcount, !any, stk + threadIdx.x, BLOCK, cnt);
         if (!any) {
             // The wave has no path in flight: count what it finished on this
             // scene, then move to the waiting chunk, the next chunk, or stop.

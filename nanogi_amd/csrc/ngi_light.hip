@@ -45,15 +45,7 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsig
             }
         }
         const bool any = __ballot(active) != 0;
-        if (QUEUE) {
-            // Full rounds of 64; the remainder once every path has ended.
-            while (qcount >= 64u || (!any && exhausted && qcount > 0u)) {
-                const uint32_t n = qcount < 64u ? qcount : 64u;
-                shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, qhead, n, stk + threadIdx.x, NGI_LT_BLOCK, cnt);
-                qhead += n;
-                qcount -= n;
-            }
-        }
+        if (QUEUE) shq_flush<FilmT, WIDTH, COUNT>(a, q, wave, qhead, qcount, !any && exhausted, stk + threadIdx.x, NGI_LT_BLOCK, cnt);
         if (!any) {
             if (exhausted) break;
             continue;

@@ -60,7 +60,18 @@ __device__ __forceinline__ void film_add(void* film, int pixel, V3<R> v) {
     if (v.z != R(0)) unsafeAtomicAdd(f + 2, (FilmT)v.z);
 }
 
-// Deferred shadow rays of the persistent engine.  Only about half of the
+
+template <int WIDTH, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, TraceHit& h, TraceCounters& c, uint32_t* stk, int stride) {
+    if (sc.num_tris == 0) {   // no geometry (and no nodes): every query misses
+        h.leaf = -1;
+        return false;
+    }
+    if (WIDTH == 8) return trace_bvh8<ANY, COUNT>(sc, r, h, c, stk, stride);
+    return trace_bvh2<ANY, COUNT>(sc, r, h, c);
+}
+
+// Deferred shadow rays of the persistent engines.  Only about half of the
 // NEE samples of a vertex need a shadow ray, so tracing them where they are
 // made runs the any-hit traversal with half of the wavefront idle.  Instead
 // each wavefront appends its shadow rays (with their contribution and pixel)
@@ -68,6 +79,7 @@ __device__ __forceinline__ void film_add(void* film, int pixel, V3<R> v) {
 // The film receives the same contributions; only the order of the FP64
 // additions changes.  Record k of slot s of wavefront w is
 // rec[(k * waves + w) * NGI_SHQ + s]: (o, tmax), (d, C.r), (C.g, C.b, pixel, -).
+// A vertex step appends at most 64 rays to a queue holding at most 63.
 #define NGI_SHQ 128
 struct ShadowQueue {
     float4* rec;
@@ -102,10 +114,18 @@ __device__ __forceinline__ void shq_trace(const RenderArgs& a, const ShadowQueue
     if (!occluded) film_add<FilmT>(a.film, __float_as_int(r2.z), V3<float>(r1.w, r2.x, r2.y));
 }
 
-template <int WIDTH, bool ANY, bool COUNT>
-__device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, TraceHit& h, TraceCounters& c, uint32_t* stk, int stride) {
-    if (WIDTH == 8) return trace_bvh8<ANY, COUNT>(sc, r, h, c, stk, stride);
-    return trace_bvh2<ANY, COUNT>(sc, r, h, c);
+// Traces the queued shadow rays in rounds of 64 (one ray per lane) while 64
+// or more are waiting, or all of them when `all` (the wave is about to stop
+// or to change scene).
+template <typename FilmT, int WIDTH, bool COUNT>
+__device__ __forceinline__ void shq_flush(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t& head, uint32_t& count,
+                                          bool all, uint32_t* stk, int stride, uint32_t* cnt) {
+    while (count >= 64u || (all && count > 0u)) {
+        const uint32_t n = count < 64u ? count : 64u;
+        shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, head, n, stk, stride, cnt);
+        head += n;
+        count -= n;
+    }
 }
 
 // Surface at a hit: p = o + d t in the shading precision (rt.hpp:2197),

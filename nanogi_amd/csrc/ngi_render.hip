@@ -96,15 +96,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(
             }
         }
         const bool any = __ballot(active) != 0;
-        if (QUEUE) {
-            // Full rounds of 64; once every path has ended, the remainder.
-            while (qcount >= 64u || (!any && qcount > 0u)) {
-                const uint32_t n = qcount < 64u ? qcount : 64u;
-                shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, qhead, n, stk + threadIdx.x, NGI_MK_BLOCK, cnt);
-                qhead += n;
-                qcount -= n;
-            }
-        }
+        if (QUEUE) shq_flush<FilmT, WIDTH, COUNT>(a, q, wave, qhead, qcount, !any, stk + threadIdx.x, NGI_MK_BLOCK, cnt);
         if (!any) break;
         bool pushed = false;
         if (active)

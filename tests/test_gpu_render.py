@@ -165,3 +165,25 @@ def test_persistent_engine_matches_megakernel(renderer):
     assert st["samples"] == st2["samples"] == 40 * 30 * 8
     assert abs(st["extension_rays"] - st2["extension_rays"]) <= 1e-3 * st2["extension_rays"]
     _crn_close(g, m, 2e-4)
+
+
+@pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
+def test_scene_without_triangles(width):
+    """A camera and a point light and nothing else: every ray misses (an
+    Embree scene with no geometry), so pt and lt see nothing and the
+    next-event estimators splat the light where it projects."""
+    B = scenes.SceneBuilder()
+    B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_POINT, Le=(2, 1, 1), light_position=(0.1, 0.2, -2.0))
+    B.camera((0, 0, 0), (0, 0, -1))
+    s = ngi.Scene.create(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64), np.zeros(0, dtype=np.int64),
+                         np.array(B.prims), normals=np.zeros((0, 3)))
+    s.build(width)
+    for r in ("pt", "ptdirect", "lt", "ltdirect", "bdpt"):
+        for eng in ENGINES:
+            p = ngi.default_params(renderer=r, width=16, height=16, sampling=ngi.SAMPLING_PER_PIXEL, spp=4,
+                                   max_num_vertices=4, seed=3, engine=eng)
+            g, st = ngi.render(s, p)
+            o = oracle.render(s, p)
+            assert st["extension_rays"] >= 0 and np.isfinite(g).all()
+            np.testing.assert_allclose(g, o, rtol=1e-5, atol=1e-9, err_msg=f"{r} {eng}")
+            assert (np.count_nonzero(o.sum(axis=2)) == 1) == (r not in ("pt", "lt")), r
