@@ -61,6 +61,27 @@ __device__ __forceinline__ void bstore(const BdptMem& m, int side, int k, uint32
     q[3] = make_float4(v.alpha.x, v.alpha.y, v.alpha.z, 0.0f);
 }
 
+// Geometry only (alpha left zero): the MIS walk never reads the throughput.
+__device__ __forceinline__ BVert bload_geo(const BdptMem& m, int side, int k, uint32_t lane) {
+    const float4* q = brec(m, side, k, lane);
+    const float4 a = q[0], b = q[1], c = q[2];
+    BVert v;
+    v.p = f3(a.x, a.y, a.z);
+    v.prim = __float_as_int(a.w);
+    v.gn = f3(b.x, b.y, b.z);
+    const int tb = __float_as_int(b.w);
+    v.type = tb & 255;
+    v.deg = (tb & 256) != 0;
+    v.n = f3(c.x, c.y, c.z);
+    v.alpha = f3(0.0f);
+    return v;
+}
+
+__device__ __forceinline__ f3 bload_pos(const BdptMem& m, int side, int k, uint32_t lane) {
+    const float4 a = brec(m, side, k, lane)[0];
+    return f3(a.x, a.y, a.z);
+}
+
 __device__ __forceinline__ BVert bload(const BdptMem& m, int side, int k, uint32_t lane) {
     const float4* q = brec(m, side, k, lane);
     const float4 a = q[0], b = q[1], c = q[2], d = q[3];
@@ -165,6 +186,12 @@ struct Conn {
         if (k == 0) v.type = NGI_PRIM_L;   // s = 0: the eye subpath's end on a light acts as the light (bdpt.hpp:150)
         return v;
     }
+    __device__ __forceinline__ BVert xg(int k) const {   // x(k) without the throughput
+        BVert v = k < s ? bload_geo(*m, 0, k, lane) : bload_geo(*m, 1, n - 1 - k, lane);
+        if (k == 0) v.type = NGI_PRIM_L;
+        return v;
+    }
+    __device__ __forceinline__ f3 xp(int k) const { return k < s ? bload_pos(*m, 0, k, lane) : bload_pos(*m, 1, n - 1 - k, lane); }
 };
 
 __device__ __forceinline__ f3 dirv(const BVert& from, const BVert& to) { return vnorm(to.p - from.p); }
@@ -205,8 +232,91 @@ __device__ __forceinline__ double pdf_eye_side(const BdptCtx& c, const Conn& q, 
     return (double)vert_pdf(c, a, k + 2 <= n - 1, wi, dirv(a, b)) * (double)gterm(a, b);
 }
 
+// c_i of cst() from the vertices around the split (vm2 = x_{i-2}.p when i >= 2,
+// vp1 = x_{i+1}.p when i + 1 < n).
+__device__ __forceinline__ f3 cst_at(const BdptCtx& c, int i, int n, f3 vm2, const BVert& vL, const BVert& vE, f3 vp1) {
+    const f3 wiL = i - 2 >= 0 ? vnorm(vm2 - vL.p) : f3(0.0f);
+    const f3 wiE = i + 1 < n ? vnorm(vp1 - vE.p) : f3(0.0f);
+    const f3 fsL = vert_eval<true>(c, vL, i - 2 >= 0, wiL, dirv(vL, vE));
+    if (is_zero(fsL)) return f3(0.0f);
+    const f3 fsE = vert_eval<false>(c, vE, i + 1 < n, wiE, dirv(vE, vL));
+    return fsL * fsE * gterm(vL, vE);
+}
+
+// mis_weight() below with each vertex of the connected path loaded once
+// per walk: a window of three vertices (and the positions one further out)
+// slides away from the connection, and every pdf and c_i is evaluated from
+// it with the same arguments, hence the same values, as the definitions
+// above.
+__device__ __forceinline__ double mis_weight_window(const BdptCtx& c, const Conn& q) {
+    const int s = q.s, n = q.n;
+    double inv = 1.0, r = 1.0;
+    if (s >= 1) {
+        // Towards the light, i = s - 1 .. 0: B = x_i, C = x_{i+1}, pC2 = x_{i+2}.p.
+        BVert B = q.xg(s - 1), C = q.xg(s);
+        f3 pC2 = s + 1 <= n - 1 ? q.xp(s + 1) : f3(0.0f);
+        for (int i = s - 1; i >= 0; --i) {
+            // pdf_eye_side(i): i <= n - 2 here (t >= 1).
+            const f3 wiC = i + 2 <= n - 1 ? vnorm(pC2 - C.p) : f3(0.0f);
+            const double pe = (double)vert_pdf(c, C, i + 2 <= n - 1, wiC, dirv(C, B)) * (double)gterm(C, B);
+            if (pe == 0.0) break;
+            double pl;
+            f3 ci;
+            BVert A;
+            if (i == 0) {
+                pl = (double)pos_pdf(c, B);
+                ci = vert_eval<false>(c, B, false, f3(0.0f), dirv(B, C)) * pos_eval_unforced(c, B);
+            } else {
+                A = q.xg(i - 1);
+                const f3 pA2 = i - 2 >= 0 ? q.xp(i - 2) : f3(0.0f);
+                const f3 wiA = i - 2 >= 0 ? vnorm(pA2 - A.p) : f3(0.0f);
+                pl = (double)vert_pdf(c, A, i - 2 >= 0, wiA, dirv(A, B)) * (double)gterm(A, B);
+                ci = cst_at(c, i, n, pA2, A, B, C.p);
+            }
+            r *= pe / pl;
+            if (!is_zero(ci)) inv += r * r;
+            if (i == 0) break;
+            pC2 = C.p;
+            C = B;
+            B = A;
+        }
+    }
+    r = 1.0;
+    if (s + 1 < n) {
+        // Towards the eye, i = s + 1 .. n - 1: A = x_{i-2}, B = x_{i-1}, pA2 = x_{i-3}.p.
+        BVert B = q.xg(s), A = B;   // A is read only once i >= 2
+        f3 pA2(0.0f);
+        if (s >= 1) A = q.xg(s - 1);
+        if (s >= 2) pA2 = q.xp(s - 2);
+        for (int i = s + 1; i < n; ++i) {
+            const int k = i - 1;   // pdf_light_side(k)
+            double pl;
+            if (k == 0) {
+                pl = (double)pos_pdf(c, B);
+            } else {
+                const f3 wiA = k - 2 >= 0 ? vnorm(pA2 - A.p) : f3(0.0f);
+                pl = (double)vert_pdf(c, A, k - 2 >= 0, wiA, dirv(A, B)) * (double)gterm(A, B);
+            }
+            if (pl == 0.0) break;
+            const BVert C = q.xg(i);
+            const f3 pC1 = i + 1 <= n - 1 ? q.xp(i + 1) : f3(0.0f);
+            // pdf_eye_side(k): k <= n - 2.
+            const f3 wiC = i + 1 <= n - 1 ? vnorm(pC1 - C.p) : f3(0.0f);
+            const double pe = (double)vert_pdf(c, C, i + 1 <= n - 1, wiC, dirv(C, B)) * (double)gterm(C, B);
+            r *= pl / pe;
+            if (!is_zero(cst_at(c, i, n, A.p, B, C, pC1))) inv += r * r;
+            pA2 = A.p;
+            A = B;
+            B = C;
+        }
+    }
+    const double w = 1.0 / inv;
+    return (w == w && inv < 1e300) ? w : 0.0;
+}
+
 // Power-heuristic weight of strategy s: 1 / sum_i (p_i / p_s)^2 over the
-// strategies with c_i != 0.  Returns 0 when p_s itself vanishes.
+// strategies with c_i != 0.  Returns 0 when p_s itself vanishes.  (The
+// definition; the kernel evaluates it through mis_weight_window.)
 __device__ __forceinline__ double mis_weight(const BdptCtx& c, const Conn& q) {
     const int s = q.s, n = q.n;
     double inv = 1.0, r = 1.0;
@@ -377,7 +487,7 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsign
                 const f3 aE = bload(mem, 1, t - 1, gl).alpha;
                 const f3 Cu = aL * cs * aE;
                 if (is_zero(Cu)) continue;
-                const double w = mis_weight(c, q);
+                const double w = mis_weight_window(c, q);
                 if (w == 0.0) continue;
                 float sel = 1.0f;   // SelectionProb (bdpt.hpp:193-208)
                 for (int i = 1; i < s - 1; ++i) sel *= rr_at(a, i);

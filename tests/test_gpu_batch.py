@@ -71,14 +71,18 @@ def test_batch_matches_oracle():
     np.testing.assert_allclose(films[0].reshape(-1, 3).mean(0), o.reshape(-1, 3).mean(0), rtol=2e-3)
 
 
-def test_batch_of_many_small_scenes_and_binary_bvh():
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_batch_of_many_small_scenes_and_binary_bvh(renderer):
+    """Tiny scenes (a wave crosses many of them, draining its shadow queue
+    each time it leaves one) on the binary BVH."""
     scenes, params = [], []
     rng = np.random.default_rng(0)
     for i in range(37):
         s = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 100 + i, 1.0, int(rng.integers(12, 600)))
         s.build(ngi.BVH_2)
         scenes.append(s)
-        params.append(_p("pt", int(rng.integers(4, 20)), int(rng.integers(4, 20)), int(rng.integers(1, 6)), seed=i, stream_id=i))
+        params.append(_p(renderer, int(rng.integers(4, 20)), int(rng.integers(4, 20)), int(rng.integers(1, 6)), seed=i,
+                         stream_id=i))
     films, ms, _ = ngi.render_batch(scenes, params)
     for k in (0, 17, 36):
         _same(films[k], ngi.render(scenes[k], params[k])[0])
