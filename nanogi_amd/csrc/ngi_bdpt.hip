@@ -9,7 +9,11 @@
 // alphaL(s) * c(s,t) * alphaE(t) (:210-333), the power-heuristic weight over
 // all strategies that could produce the same path (:352-372) and the
 // Russian-roulette selection probability (:193-208); the result goes to the
-// pixel the camera end of the path projects to.
+// pixel the camera end of the path projects to.  The strategies of all the
+// samples of a wavefront are pooled and evaluated 64 at a time by whichever
+// lane is free, reading the owning lane's records, so the connection rays and
+// weights run with the whole wavefront busy although samples differ in their
+// number of strategies.
 //
 // The weight is computed differently from the reference's
 // EvaluatePowerHeuristicsMISWeightOpt, which recomputes every p_i(x) from
@@ -337,8 +341,78 @@ __device__ __forceinline__ double mis_weight(const BdptCtx& c, const Conn& q) {
     return (w == w && inv < 1e300) ? w : 0.0;   // a strategy with p_s = 0 (never sampled) gets no weight
 }
 
+// Strategies (s, t = n - s >= 1) of a sample with nL light and nE eye
+// vertices and at most nmax vertices per path, in the reference's order.
+__device__ __forceinline__ int strategies_of_length(int n, int nL, int nE) {
+    const int lo = n - nE > 0 ? n - nE : 0, hi = nL < n - 1 ? nL : n - 1;
+    return hi >= lo ? hi - lo + 1 : 0;
+}
+
+__device__ __forceinline__ uint32_t strategy_count(int nL, int nE, int nmax) {
+    uint32_t k = 0;
+    for (int n = 2; n <= nmax; ++n) k += (uint32_t)strategies_of_length(n, nL, nE);
+    return k;
+}
+
+// The k-th strategy: its path length n and light-vertex count s.
+__device__ __forceinline__ void strategy_at(uint32_t k, int nL, int nE, int nmax, int& n, int& s) {
+    for (n = 2; n <= nmax; ++n) {
+        const uint32_t c = (uint32_t)strategies_of_length(n, nL, nE);
+        if (k < c) {
+            s = (n - nE > 0 ? n - nE : 0) + (int)k;
+            return;
+        }
+        k -= c;
+    }
+    n = s = 0;   // not reached for k below strategy_count
+}
+
+// One strategy of the sample whose records live at `owner`: the shadow ray
+// between the subpath ends (Connect, bdpt.hpp:139-191), the unweighted
+// contribution alphaL * c(s,t) * alphaE, the power-heuristic weight and the
+// selection probability; the result is splatted where the camera end of the
+// path projects.
 template <typename FilmT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
+__device__ __forceinline__ void eval_strategy(const RenderArgs& a, const BdptCtx& c, const BdptMem& mem, uint32_t owner, int s, int n,
+                                              uint32_t* my_stk, uint32_t* cnt) {
+    const int t = n - s;
+    if (n < 2 || t < 1) return;
+    Conn q{&mem, owner, s, n};
+    if (s == 0) {
+        const BVert e = bload(mem, 1, t - 1, owner);
+        if (!(c.prims[e.prim].type & NGI_PRIM_L)) return;
+    } else {
+        const BVert l = bload(mem, 0, s - 1, owner), e = bload(mem, 1, t - 1, owner);
+        const f3 dv = e.p - l.p;
+        const float len = sqrtf(dot(dv, dv));
+        const f3 d = dv / len;
+        TraceRay sr{l.p.x, l.p.y, l.p.z, d.x, d.y, d.z, NGI_EPS_F, len * (1.0f - NGI_EPS_F)};   // Scene::Visible
+        TraceHit sh;
+        TraceCounters tc{0, 0};
+        const bool occluded = trace<WIDTH, true, COUNT>(*c.sc, sr, sh, tc, my_stk, NGI_BD_BLOCK);
+        cnt[CNT_SHADOW_RAYS]++;
+        if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
+        if (occluded) return;
+    }
+    const f3 cs = cst(c, q, s);
+    if (is_zero(cs)) return;
+    const f3 aL = s > 0 ? bload(mem, 0, s - 1, owner).alpha : f3(1.0f);
+    const f3 aE = bload(mem, 1, t - 1, owner).alpha;
+    const f3 Cu = aL * cs * aE;
+    if (is_zero(Cu)) return;
+    const double w = mis_weight_window(c, q);
+    if (w == 0.0) return;
+    float sel = 1.0f;   // SelectionProb (bdpt.hpp:193-208)
+    for (int i = 1; i < s - 1; ++i) sel *= rr_at(a, i);
+    for (int i = t - 2; i >= 1; --i) sel *= rr_at(a, i);
+    const BVert cam_v = q.x(n - 1), prev = q.x(n - 2);
+    float rx = 0.0f, ry = 0.0f;
+    camera_importance(c.cam, dirv(cam_v, prev), rx, ry);
+    film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), Cu * (float)(w / (double)sel));
+}
+
+template <typename FilmT, int WIDTH, bool COUNT>
+__global__ void __launch_bounds__(NGI_BD_BLOCK, 3) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     const DevScene& sc = a.sc;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
@@ -359,143 +433,136 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK) bdpt_kernel(RenderArgs a, unsign
         if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(act));
         base = __shfl(base, leader);
         const unsigned long long local = base + (unsigned long long)__popcll(act & ((1ull << lane) - 1ull));
-        if (local >= a.path_count) break;
-        uint32_t c0, c1;
-        int pixel;
-        path_coords(a, a.path_begin + local, c0, c1, pixel);
-        cnt[CNT_SAMPLES]++;
-        // ---- subpaths ----
+        // Lanes without a path id left stay in the loop with no strategies, so
+        // every lane of the wavefront takes part in the strategy rounds.
+        const bool have = local < a.path_count;
+        if (__ballot(have) == 0) break;
         int nsub[2] = {0, 0};
-        for (int side = 0; side < 2; ++side) {   // 0: light (LE), 1: eye (EL)
-            const uint32_t stream = side == 0 ? NGI_STREAM_NEE : NGI_STREAM_EXT;
-            BVert pv;
-            f3 ppv_p(0.0f);
-            if (side == 0) {
-                if (sc.num_lights == 0) continue;
-                const U4 r0 = rng4(c0, c1, 0u, stream, a.seed, a.stream_id);
-                const int li = pick_light(sc, u24f(r0.x));
-                const DevLight<float>& L = c.lights[li];
-                pv.prim = L.prim;
-                pv.type = NGI_PRIM_L;
-                if (L.type == NGI_LIGHT_AREA) {
-                    area_light_point(sc, L, u24f(r0.y), u24f(r0.z), pv.p, pv.gn);
-                    pv.n = pv.gn;
-                    pv.deg = false;
-                } else if (L.type == NGI_LIGHT_POINT) {
-                    pv.p = f3(L.pos[0], L.pos[1], L.pos[2]);
+        if (have) {
+            uint32_t c0, c1;
+            int pixel;
+            path_coords(a, a.path_begin + local, c0, c1, pixel);
+            cnt[CNT_SAMPLES]++;
+            // ---- subpaths ----
+            for (int side = 0; side < 2; ++side) {   // 0: light (LE), 1: eye (EL)
+                const uint32_t stream = side == 0 ? NGI_STREAM_NEE : NGI_STREAM_EXT;
+                BVert pv;
+                f3 ppv_p(0.0f);
+                if (side == 0) {
+                    if (sc.num_lights == 0) continue;
+                    const U4 r0 = rng4(c0, c1, 0u, stream, a.seed, a.stream_id);
+                    const int li = pick_light(sc, u24f(r0.x));
+                    const DevLight<float>& L = c.lights[li];
+                    pv.prim = L.prim;
+                    pv.type = NGI_PRIM_L;
+                    if (L.type == NGI_LIGHT_AREA) {
+                        area_light_point(sc, L, u24f(r0.y), u24f(r0.z), pv.p, pv.gn);
+                        pv.n = pv.gn;
+                        pv.deg = false;
+                    } else if (L.type == NGI_LIGHT_POINT) {
+                        pv.p = f3(L.pos[0], L.pos[1], L.pos[2]);
+                        pv.gn = pv.n = f3(0.0f);
+                        pv.deg = true;
+                    } else {
+                        const f3 d(L.dir[0], L.dir[1], L.dir[2]);
+                        const Frame<float> fr = make_frame(d);
+                        float dx, dy;
+                        concentric_disk(u24f(r0.y), u24f(r0.z), dx, dy);
+                        pv.p = f3(L.pos[0], L.pos[1], L.pos[2]) - d * L.radius + (fr.s * (dx * L.radius) + fr.t * (dy * L.radius));
+                        pv.gn = pv.n = d;
+                        pv.deg = false;
+                    }
+                    pv.alpha = f3(1.0f / pos_pdf(c, pv));   // EvaluatePosition(.., true) = 1
+                } else {
+                    pv.prim = sc.sensor_prim;
+                    pv.type = NGI_PRIM_E;
+                    pv.p = c.cam.eye;
                     pv.gn = pv.n = f3(0.0f);
                     pv.deg = true;
-                } else {
-                    const f3 d(L.dir[0], L.dir[1], L.dir[2]);
-                    const Frame<float> fr = make_frame(d);
-                    float dx, dy;
-                    concentric_disk(u24f(r0.y), u24f(r0.z), dx, dy);
-                    pv.p = f3(L.pos[0], L.pos[1], L.pos[2]) - d * L.radius + (fr.s * (dx * L.radius) + fr.t * (dy * L.radius));
-                    pv.gn = pv.n = d;
-                    pv.deg = false;
+                    pv.alpha = f3(1.0f);
                 }
-                pv.alpha = f3(1.0f / pos_pdf(c, pv));   // EvaluatePosition(.., true) = 1
-            } else {
-                pv.prim = sc.sensor_prim;
-                pv.type = NGI_PRIM_E;
-                pv.p = c.cam.eye;
-                pv.gn = pv.n = f3(0.0f);
-                pv.deg = true;
-                pv.alpha = f3(1.0f);
-            }
-            bstore(mem, side, 0, gl, pv);
-            int nv = 1;
-            for (int step = 1; step < maxv; ++step) {
-                const U4 r = rng4(c0, c1, (uint32_t)step, stream, a.seed, a.stream_id);
-                const float u0 = u24f(r.x), u1 = u24f(r.y), uc = u24f(r.z), urr = u24f(r.w);
-                const bool has_wi = step >= 2;
-                const f3 wi = has_wi ? vnorm(ppv_p - pv.p) : f3(0.0f);
-                f3 wo, wgt;
-                if (pv.type == NGI_PRIM_L) {
-                    const DevLight<float>& L = light_of(c, pv);
-                    const f3 Le(L.Le[0], L.Le[1], L.Le[2]);
-                    if (L.type == NGI_LIGHT_AREA) {
-                        wo = to_world(make_frame(pv.n), cosine_hemisphere(u0, u1));
-                        if (dot(pv.n, wo) <= 0.0f) break;
-                        wgt = Le * (float)NGI_PI;
-                    } else if (L.type == NGI_LIGHT_POINT) {
-                        wo = uniform_sphere(u0, u1);
-                        wgt = Le * (float)(4.0 * NGI_PI);
+                bstore(mem, side, 0, gl, pv);
+                int nv = 1;
+                for (int step = 1; step < maxv; ++step) {
+                    const U4 r = rng4(c0, c1, (uint32_t)step, stream, a.seed, a.stream_id);
+                    const float u0 = u24f(r.x), u1 = u24f(r.y), uc = u24f(r.z), urr = u24f(r.w);
+                    const bool has_wi = step >= 2;
+                    const f3 wi = has_wi ? vnorm(ppv_p - pv.p) : f3(0.0f);
+                    f3 wo, wgt;
+                    if (pv.type == NGI_PRIM_L) {
+                        const DevLight<float>& L = light_of(c, pv);
+                        const f3 Le(L.Le[0], L.Le[1], L.Le[2]);
+                        if (L.type == NGI_LIGHT_AREA) {
+                            wo = to_world(make_frame(pv.n), cosine_hemisphere(u0, u1));
+                            if (dot(pv.n, wo) <= 0.0f) break;
+                            wgt = Le * (float)NGI_PI;
+                        } else if (L.type == NGI_LIGHT_POINT) {
+                            wo = uniform_sphere(u0, u1);
+                            wgt = Le * (float)(4.0 * NGI_PI);
+                        } else {
+                            wo = f3(L.dir[0], L.dir[1], L.dir[2]);
+                            wgt = Le;
+                        }
+                    } else if (pv.type == NGI_PRIM_E) {
+                        wo = camera_sample<float>(a, c.cam, u0, u1, pixel);
+                        wgt = f3(1.0f);   // pinhole fs / pdf = 1
                     } else {
-                        wo = f3(L.dir[0], L.dir[1], L.dir[2]);
-                        wgt = Le;
+                        const bool ok = side == 0 ? bsdf_sample<float, true>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt)
+                                                  : bsdf_sample<float, false>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt);
+                        if (!ok) break;
                     }
-                } else if (pv.type == NGI_PRIM_E) {
-                    wo = camera_sample<float>(a, c.cam, u0, u1, pixel);
-                    wgt = f3(1.0f);   // pinhole fs / pdf = 1
-                } else {
-                    const bool ok = side == 0 ? bsdf_sample<float, true>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt)
-                                              : bsdf_sample<float, false>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt);
-                    if (!ok) break;
-                }
-                TraceRay er{pv.p.x, pv.p.y, pv.p.z, wo.x, wo.y, wo.z, NGI_EPS_F, FLT_MAX};
-                TraceHit h;
-                TraceCounters tc{0, 0};
-                const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, my_stk, NGI_BD_BLOCK);
-                cnt[CNT_EXT_RAYS]++;
-                if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
-                if (!hit) break;
-                int hp;
-                const Surf<float> ns = hit_surface<float>(sc, pv.p, wo, h, hp);
-                BVert v;
-                v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
-                v.prim = hp;
-                v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
-                v.alpha = pv.alpha * wgt;
-                bstore(mem, side, step, gl, v);
-                nv = step + 1;
-                const float q = rr_at(a, step);
-                if (q < 1.0f && urr > q) break;   // the vertex is kept (bdpt.hpp:118-123)
-                ppv_p = pv.p;
-                pv = v;
-            }
-            nsub[side] = nv;
-        }
-        // ---- strategies ----
-        const int nL = nsub[0], nE = nsub[1];
-        for (int n = 2; n <= nL + nE; ++n) {
-            if (a.max_nv >= 0 && n > a.max_nv) break;
-            const int s_lo = n - nE > 0 ? n - nE : 0, s_hi = nL < n ? nL : n;
-            for (int s = s_lo; s <= s_hi; ++s) {
-                const int t = n - s;
-                if (t == 0) continue;   // a pinhole is never hit
-                Conn q{&mem, gl, s, n};
-                if (s == 0) {
-                    const BVert e = bload(mem, 1, t - 1, gl);
-                    if (!(c.prims[e.prim].type & NGI_PRIM_L)) continue;
-                } else {
-                    const BVert l = bload(mem, 0, s - 1, gl), e = bload(mem, 1, t - 1, gl);
-                    const f3 dv = e.p - l.p;
-                    const float len = sqrtf(dot(dv, dv));
-                    const f3 d = dv / len;
-                    TraceRay sr{l.p.x, l.p.y, l.p.z, d.x, d.y, d.z, NGI_EPS_F, len * (1.0f - NGI_EPS_F)};   // Scene::Visible
-                    TraceHit sh;
+                    TraceRay er{pv.p.x, pv.p.y, pv.p.z, wo.x, wo.y, wo.z, NGI_EPS_F, FLT_MAX};
+                    TraceHit h;
                     TraceCounters tc{0, 0};
-                    const bool occluded = trace<WIDTH, true, COUNT>(sc, sr, sh, tc, my_stk, NGI_BD_BLOCK);
-                    cnt[CNT_SHADOW_RAYS]++;
-                    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
-                    if (occluded) continue;
+                    const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, my_stk, NGI_BD_BLOCK);
+                    cnt[CNT_EXT_RAYS]++;
+                    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+                    if (!hit) break;
+                    int hp;
+                    const Surf<float> ns = hit_surface<float>(sc, pv.p, wo, h, hp);
+                    BVert v;
+                    v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
+                    v.prim = hp;
+                    v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                    v.alpha = pv.alpha * wgt;
+                    bstore(mem, side, step, gl, v);
+                    nv = step + 1;
+                    const float q = rr_at(a, step);
+                    if (q < 1.0f && urr > q) break;   // the vertex is kept (bdpt.hpp:118-123)
+                    ppv_p = pv.p;
+                    pv = v;
                 }
-                const f3 cs = cst(c, q, s);
-                if (is_zero(cs)) continue;
-                const f3 aL = s > 0 ? bload(mem, 0, s - 1, gl).alpha : f3(1.0f);
-                const f3 aE = bload(mem, 1, t - 1, gl).alpha;
-                const f3 Cu = aL * cs * aE;
-                if (is_zero(Cu)) continue;
-                const double w = mis_weight_window(c, q);
-                if (w == 0.0) continue;
-                float sel = 1.0f;   // SelectionProb (bdpt.hpp:193-208)
-                for (int i = 1; i < s - 1; ++i) sel *= rr_at(a, i);
-                for (int i = t - 2; i >= 1; --i) sel *= rr_at(a, i);
-                const BVert cam_v = q.x(n - 1), prev = q.x(n - 2);
-                float rx = 0.0f, ry = 0.0f;
-                camera_importance(c.cam, dirv(cam_v, prev), rx, ry);
-                film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), Cu * (float)(w / (double)sel));
+                nsub[side] = nv;
+            }
+        }
+        // Every lane reads the other lanes' vertex records below.
+        __threadfence_block();
+        // ---- strategies ----
+        // The strategies (s, t) of all the wavefront's samples form one pool,
+        // evaluated 64 at a time: lane j of a round takes pool entry base + j,
+        // which belongs to lane `owner` (whose records it reads) as its k-th
+        // strategy in the reference's order (n ascending, then s).  A round
+        // keeps every lane on a connection regardless of how many strategies
+        // each sample has.
+        const int nmax = (a.max_nv >= 0 && a.max_nv < nsub[0] + nsub[1]) ? a.max_nv : nsub[0] + nsub[1];
+        const uint32_t K = have ? strategy_count(nsub[0], nsub[1], nmax) : 0u;
+        uint32_t incl = K;   // inclusive prefix sum over the lanes
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t g = base + lane < total ? base + lane : total - 1;
+            int owner = 0;   // the first lane whose prefix sum exceeds g
+            for (int step = 32; step > 0; step >>= 1)
+                if (__shfl(incl, owner + step - 1) <= g) owner += step;
+            const uint32_t k = g - (__shfl(incl, owner) - __shfl(K, owner));
+            const int oL = __shfl(nsub[0], owner), oE = __shfl(nsub[1], owner), om = __shfl(nmax, owner);
+            if (base + lane < total) {
+                int n = 0, s = 0;
+                strategy_at(k, oL, oE, om, n, s);
+                eval_strategy<FilmT, WIDTH, COUNT>(a, c, mem, gl - lane + (uint32_t)owner, s, n, my_stk, cnt);
             }
         }
     }
