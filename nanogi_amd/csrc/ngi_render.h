@@ -44,13 +44,18 @@ inline int persistent_grid(const void* kern, int block, size_t lds, uint64_t pat
 }
 
 // Device scratch owned by a session and grown on demand (per-lane bdpt
-// subpath records, per-wavefront shadow queues).
+// subpath records, per-wavefront shadow queues).  Growing waits for the
+// session's stream first: an earlier launch may still be using the buffer.
 struct LaneMem {
     void** ptr;
     size_t* bytes;
+    hipStream_t stream;
     int reserve(size_t n) const {
         if (*bytes >= n) return NGI_OK;
-        if (*ptr) NGI_HIP_CHECK(hipFree(*ptr));
+        if (*ptr) {
+            NGI_HIP_CHECK(hipStreamSynchronize(stream));
+            NGI_HIP_CHECK(hipFree(*ptr));
+        }
         *ptr = nullptr;
         *bytes = 0;
         if (hipMalloc(ptr, n) != hipSuccess) {

@@ -365,7 +365,7 @@ int Session::launch_pk_engine(const RenderArgs& a, Lanes ln) {
     unsigned long long* next = reinterpret_cast<unsigned long long*>(next_path);
     const int width = scene->dev->view.bvh_width;
     const bool count = p.collect_counters != 0;
-    LaneMem mem{&lane_mem, &lane_bytes};
+    LaneMem mem{&lane_mem, &lane_bytes, stream()};
     if (p.renderer == NGI_RENDERER_BDPT) return launch_bdpt(a, next, mem, film64, width, count, stream(), ln);
     if (is_light_tracer(p.renderer))
         return launch_light(a, next, mem, p.precision, p.renderer == NGI_RENDERER_LTDIRECT, width, count, stream(), ln);

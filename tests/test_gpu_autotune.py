@@ -80,3 +80,23 @@ def test_autotune_reference_uniform_sampling():
     assert tuned.num_samples == 50000 and len(trials) == 7
     f, st = ngi.render(s, tuned)
     assert st["samples"] == 50000 and np.isfinite(f).all() and f.mean() > 0
+
+
+@pytest.mark.parametrize("renderer", ["ptdirect", "ltdirect", "bdpt"])
+def test_scratch_grows_between_queued_launches(renderer):
+    """A one-block launch, then a full-device launch queued behind it without
+    a synchronisation: the session's shadow queues / subpath records grow in
+    between, and the film equals one launch of all the samples."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _p(renderer, 16, 16, 1, 6)
+    a = ngi.Session(s, p)
+    a.render(1)       # 256 paths: one block of lanes
+    a.render(255)     # 65,280 paths: every resident block
+    fa = a.film()
+    a.close()
+    b = ngi.Session(s, p)
+    b.render(256)
+    fb = b.film()
+    b.close()
+    assert np.abs(fb).max() > 0
+    _same(fa, fb)
