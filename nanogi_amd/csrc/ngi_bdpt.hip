@@ -586,9 +586,13 @@ static int launch_bd(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     BdptMem m;
     m.lanes = grid * NGI_BD_BLOCK;
     m.maxv = (a.max_nv < 0 || a.max_nv > NGI_BDPT_MAX_VERTICES) ? NGI_BDPT_MAX_VERTICES : a.max_nv;
+    // Every sample stores its subpaths' first vertices (the kernel has no
+    // strategy for limits of one vertex or none, but still writes them).
+    m.maxv = std::max(m.maxv, 1);
     const size_t bytes = (size_t)2 * m.maxv * m.lanes * 4 * sizeof(float4);
     if ((s = mem.reserve(bytes))) return s;
     m.v = reinterpret_cast<float4*>(*mem.ptr);
+    if (!m.v) return fail(NGI_ERR_DEVICE, "bdpt: no subpath records");
     kern<<<grid, NGI_BD_BLOCK, lds, st>>>(a, next, m);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;

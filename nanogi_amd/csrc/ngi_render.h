@@ -51,6 +51,7 @@ struct LaneMem {
     size_t* bytes;
     hipStream_t stream;
     int reserve(size_t n) const {
+        if (n == 0) return fail(NGI_ERR_INVALID_ARG, "empty per-lane record buffer");
         if (*bytes >= n) return NGI_OK;
         if (*ptr) {
             NGI_HIP_CHECK(hipStreamSynchronize(stream));

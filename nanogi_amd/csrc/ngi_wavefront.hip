@@ -65,9 +65,9 @@ __device__ __forceinline__ void store_shadow(float4* q, uint32_t slot, const Tra
 }
 
 __device__ __forceinline__ void store_path(float4* ray, float4* st, uint32_t slot, f3 o, f3 d, f3 T, int pixel, uint32_t c0,
-                                           uint32_t c1, int nv, int prim, int type) {
+                                           uint32_t c1, int nv, int prim, int type, float tmax = FLT_MAX) {
     ray[2 * (size_t)slot + 0] = make_float4(o.x, o.y, o.z, NGI_EPS_F);
-    ray[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
+    ray[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, tmax);
     st[2 * (size_t)slot + 0] = make_float4(T.x, T.y, T.z, __int_as_float(pixel));
     st[2 * (size_t)slot + 1] = __builtin_bit_cast(float4, make_uint4(c0, c1, (uint32_t)nv, ((uint32_t)prim << 5) | (uint32_t)type));
 }
@@ -77,7 +77,11 @@ template <bool DIRECT>
 __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_generate(RenderArgs a, uint64_t path_base, uint32_t n, uint32_t slot_base,
                                                            float4* ray, float4* st, float4* shadow, uint32_t* counts) {
     const uint32_t i = blockIdx.x * NGI_WF_BLOCK + threadIdx.x;
-    const bool active = i < n;
+    // The camera vertex is vertex 1: with a limit of one vertex or none the
+    // path ends before it does anything (src/nanogi.cpp:485-489).
+    const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
+    const bool active = i < n && max_nv > 1;
+    const bool dead = i < n && !active;   // occupies its queue slot with a ray that hits nothing
     const DevScene& sc = a.sc;
     const Cam<float> cam = load_cam<float>(sc.cam);
     uint32_t c0 = 0, c1 = 0;
@@ -101,6 +105,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_generate(RenderArgs a, uint64
         const uint32_t slot = wave_push(counts + 1, emit);
         if (emit) store_shadow(shadow, slot, sr, target, C);
     }
+    if (dead) store_path(ray, st, slot_base + i, cam.eye, f3(0.0f, 0.0f, 1.0f), f3(0.0f), 0, 0, 0, 1, sc.sensor_prim, NGI_PRIM_E, -1.0f);
     if (!active) return;
     const U4 rx = rng4(c0, c1, 1u, NGI_STREAM_EXT, a.seed, a.stream_id);
     const f3 wo = camera_sample<float>(a, cam, u24f(rx.x), u24f(rx.y), pixel);
@@ -237,6 +242,9 @@ static int run_wavefront(Session& S, WavefrontState& W, void* target, uint64_t t
     const uint64_t id_base = S.p.sampling == NGI_SAMPLING_PER_PIXEL ? 0 : S.paths_done;
     uint32_t live = 0;          // paths in the current input queue
     int cur = 0;                // input queue index
+    // Limits of one vertex or none: new paths end at the camera and their
+    // placeholder rays are not counted as extension rays.
+    const bool camera_only = a.max_nv >= 0 && a.max_nv <= 1;
     for (;;) {
         const uint32_t n_new = (uint32_t)std::min<uint64_t>(C - live, total - next);
         const uint32_t n = live + n_new;
@@ -265,7 +273,7 @@ static int run_wavefront(Session& S, WavefrontState& W, void* target, uint64_t t
                                                                                                            target, counters);
             if ((rc = S.timed_end(tl))) return rc;
         }
-        wf_finish<<<1, 1, 0, st>>>(W.counts, counters, n, n_new);
+        wf_finish<<<1, 1, 0, st>>>(W.counts, counters, camera_only ? live : n, n_new);
         NGI_HIP_CHECK(hipGetLastError());
         NGI_HIP_CHECK(hipMemcpyAsync(W.host_counts, W.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         NGI_HIP_CHECK(hipStreamSynchronize(st));

@@ -187,3 +187,21 @@ def test_scene_without_triangles(width):
             assert st["extension_rays"] >= 0 and np.isfinite(g).all()
             np.testing.assert_allclose(g, o, rtol=1e-5, atol=1e-9, err_msg=f"{r} {eng}")
             assert (np.count_nonzero(o.sum(axis=2)) == 1) == (r not in ("pt", "lt")), r
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect", "lt", "ltdirect", "bdpt"])
+def test_shortest_path_limits(renderer):
+    """-m 0 and -m 1 render nothing; -m 2 and -m 3 are the direct-view and
+    one-bounce cases, where the GPU matches the oracle to the usual bar."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    engines = ENGINES if renderer in ("pt", "ptdirect") else [ngi.ENGINE_PERSISTENT]
+    for m in (0, 1, 2, 3):
+        for eng in engines:
+            p = ngi.default_params(renderer=renderer, width=16, height=16, sampling=ngi.SAMPLING_PER_PIXEL, spp=8,
+                                   max_num_vertices=m, seed=2, engine=eng)
+            g, _ = ngi.render(s, p)
+            o = oracle.render(s, p)
+            if m <= 1 or not o.any():
+                assert not g.any(), (m, eng)
+                continue
+            _crn_close(g, o, 2e-4, frac=0.97)
