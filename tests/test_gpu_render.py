@@ -205,3 +205,34 @@ def test_shortest_path_limits(renderer):
                 assert not g.any(), (m, eng)
                 continue
             _crn_close(g, o, 2e-4, frac=0.97)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_every_pixel_gets_its_samples_at_odd_sizes(engine):
+    """Camera rays only (-m 2) in a closed box of emitters with reflectance 0:
+    every sample adds exactly Le to its own pixel, so any pixel the sample
+    order (8x8 tiles per wavefront) skipped or visited twice would stand out."""
+    Le = (1.0, 0.5, 0.25)
+    s = scenes.furnace(Le, (0.0, 0.0, 0.0))
+    for w, h in ((1, 1), (7, 3), (13, 9), (65, 17)):
+        g, st = ngi.render(s, _params("pt", w, h, 3, 2, engine=engine))
+        assert st["samples"] == w * h * 3
+        assert (g <= np.array(Le) * (1 + 1e-6)).all(), (w, h)
+        assert np.isclose(g, Le, rtol=1e-6).all(axis=2).mean() >= 0.99, (w, h)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_vertex_cap_in_a_lossless_furnace(engine):
+    """-m -1 without roulette in a closed box of reflectance 1: a path runs to
+    the 1024-vertex cap and sees the walls' emission 1023 times.  (About one
+    bounce in 10^4 leaves the box: a hit within t_min = 1e-4 of an edge can
+    start its next ray past the neighbouring wall, as in the reference, so
+    some paths end early; none may go past the cap.)"""
+    Le = (1.0, 0.5, 0.25)
+    s = scenes.furnace(Le, (1.0, 1.0, 1.0))
+    g, st = ngi.render(s, _params("pt", 4, 4, 2, -1, engine=engine))
+    expect = scenes.furnace_radiance(Le, (1.0, 1.0, 1.0), 1024)
+    assert (g <= expect * (1 + 1e-6)).all()
+    np.testing.assert_allclose(g.reshape(-1, 3).max(axis=0), expect, rtol=1e-5)
+    assert np.isclose(g, expect, rtol=1e-5).all(axis=2).mean() >= 0.5
+    assert 4 * 4 * 2 * 512 <= st["extension_rays"] <= 4 * 4 * 2 * 1023
