@@ -125,3 +125,31 @@ def test_autograd_harness_and_inverse_rendering_descends():
         losses.append(float(loss.detach()))
     assert losses[-1] < 0.5 * losses[0], losses
     assert float((refl - refl0)[rmask].abs().mean()) < 0.5 * float((0.5 - refl0[rmask]).abs().mean())
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_beyond_the_lds_accumulators():
+    """324 floor tiles, each its own primitive: the first 256 primitives'
+    gradients are summed in LDS, the others go straight to global FP64
+    atomics; both must agree with the oracle's gradient."""
+    from tests import scenes
+    B = scenes.SceneBuilder()
+    rng = np.random.default_rng(11)
+    n = 18
+    for i in range(n):
+        for j in range(n):
+            pid = B.prim(type=ngi.PRIM_D, D_R=tuple(rng.uniform(0.2, 0.9, 3)))
+            B.rect(pid, (-1 + (2 * i + 1) / n, 0, -1 + (2 * j + 1) / n), (0, 1, 0), (1, 0, 0), 1 / n, 1 / n)
+    lp = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=(6.0, 5.0, 4.0))
+    B.rect(lp, (0, 1.5, 0), (0, -1, 0), (1, 0, 0), 0.3, 0.3)
+    B.camera((0, 1.2, 0.01), (0, 0, 0), up=(0, 0, -1), fov=90.0)
+    s = B.scene()
+    assert len(B.prims) > 300
+    p = _params("ptdirect", 24, 24, 128, 3, -1)
+    p.engine = ngi.ENGINE_PERSISTENT
+    g = np.random.default_rng(4).normal(size=(24, 24, 3))
+    gR, gL, _ = ngi.render_adjoint(s, p, g)
+    oR, oL = oracle.render_adjoint(s, p, g)
+    for mine, ref in ((gR[:256], oR[:256]), (gR[256:], oR[256:]), (gL, oL)):
+        assert np.linalg.norm(ref) > 0
+        assert np.linalg.norm(mine - ref) <= 2e-2 * np.linalg.norm(ref)
