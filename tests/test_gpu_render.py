@@ -236,3 +236,20 @@ def test_vertex_cap_in_a_lossless_furnace(engine):
     np.testing.assert_allclose(g.reshape(-1, 3).max(axis=0), expect, rtol=1e-5)
     assert np.isclose(g, expect, rtol=1e-5).all(axis=2).mean() >= 0.5
     assert 4 * 4 * 2 * 512 <= st["extension_rays"] <= 4 * 4 * 2 * 1023
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect", "lt", "ltdirect", "bdpt"])
+def test_reference_sampling_with_few_paths(renderer):
+    """Reference-uniform sampling with 1, 63 and 1001 paths (less than one
+    wavefront, one short of a wavefront, a ragged launch): every renderer
+    draws the oracle's paths and normalises by W*H/N."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    for n in (1, 63, 1001):
+        p = ngi.default_params(renderer=renderer, width=16, height=16, num_samples=n, max_num_vertices=5, seed=9,
+                               engine=ngi.ENGINE_PERSISTENT)
+        g, st = ngi.render(s, p)
+        o = oracle.render(s, p)
+        assert st["samples"] == n
+        err = np.abs(g - o).max(axis=2)
+        bad = err > 2e-4 * (np.abs(o).max(axis=2) + 1e-3 * max(np.abs(o).max(), 1e-30))
+        assert bad.sum() <= max(1, n // 200), (n, int(bad.sum()))
