@@ -144,3 +144,30 @@ def test_batch_fixed_launch_shapes_and_plain_launches(block):
 def ctypes_copy(dst, src):
     for name, _ in src._fields_:
         setattr(dst, name, getattr(src, name))
+
+
+def test_batch_with_an_empty_scene_and_single_path_scenes():
+    """A scene without triangles (a camera and a point light), scenes of one
+    path and a one-pixel image between ordinary ones: each film equals its
+    single-scene render, and the empty scene shows only the light's splat."""
+    from tests import scenes as sc
+    B = sc.SceneBuilder()
+    B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_POINT, Le=(2, 1, 1), light_position=(0.1, 0.2, -2.0))
+    B.camera((0, 0, 0), (0, 0, -1))
+    empty = ngi.Scene.create(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64), np.zeros(0, dtype=np.int64),
+                             np.array(B.prims), normals=np.zeros((0, 3)))
+    empty.build(ngi.BVH_8)
+    room = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 9, 1.0, 500)
+    room.build(ngi.BVH_8)
+    scenes = [room, empty, room, room, empty]
+    params = [_p("ptdirect", 24, 24, 4, seed=1, stream_id=0),
+              _p("ptdirect", 16, 16, 8, seed=2, stream_id=1),
+              _p("ptdirect", 1, 1, 1, seed=3, stream_id=2),
+              _p("ptdirect", 8, 8, 1, seed=4, stream_id=3, sampling=ngi.SAMPLING_REFERENCE_UNIFORM, num_samples=1),
+              _p("ptdirect", 4, 4, 2, seed=5, stream_id=4)]
+    films, ms, st = ngi.render_batch(scenes, params)
+    assert st["samples"] == 24 * 24 * 4 + 16 * 16 * 8 + 1 + 1 + 4 * 4 * 2
+    for s, p, f in zip(scenes, params, films):
+        ref, _ = ngi.render(s, p)
+        np.testing.assert_allclose(f, ref, rtol=2e-4, atol=1e-6 * max(np.abs(ref).max(), 1.0))
+    assert np.count_nonzero(films[1].sum(axis=2)) == 1
