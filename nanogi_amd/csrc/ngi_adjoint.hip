@@ -33,6 +33,12 @@
 namespace ngi {
 
 #define NGI_ADJ_BLOCK 256
+// Minimum waves per SIMD for the adjoint kernel's register allocation: 4
+// (128 VGPRs) measured fastest on the nave (38.2 ms per 64-spp replay against
+// 44.8 ms unconstrained at 140 VGPRs / 3 waves, 38.6 ms at 5 waves).
+#ifndef NGI_ADJ_WAVES
+#define NGI_ADJ_WAVES 4
+#endif
 #define NGI_ADJ_LDS_PRIMS 256
 
 struct AdjArgs {
@@ -53,7 +59,7 @@ __device__ __forceinline__ void grad_add(double* lds, double* glob, int nlds, in
 }
 
 template <bool DIRECT, int WIDTH>
-__global__ void __launch_bounds__(NGI_ADJ_BLOCK) adjoint_kernel(RenderArgs a, AdjArgs g, unsigned long long* next) {
+__global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(RenderArgs a, AdjArgs g, unsigned long long* next) {
     extern __shared__ double adj_smem[];
     const int nlds = min(g.nprims, NGI_ADJ_LDS_PRIMS);
     double* sR = adj_smem;

@@ -34,6 +34,12 @@
 namespace ngi {
 
 #define NGI_BD_BLOCK 256
+// Minimum waves per SIMD for the bdpt kernel's register allocation: 4
+// (128 VGPRs, a little spilled) measured fastest against 3 (160 VGPRs) and
+// 5: +9% on the Cornell box, +16% on the nave.
+#ifndef NGI_BD_WAVES
+#define NGI_BD_WAVES 4
+#endif
 // Vertices per subpath; the reference has no cap with -m -1 (with Russian
 // roulette at 0.5 a longer subpath has probability 2^-62).
 #define NGI_BDPT_MAX_VERTICES 64
@@ -412,7 +418,7 @@ __device__ __forceinline__ void eval_strategy(const RenderArgs& a, const BdptCtx
 }
 
 template <typename FilmT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_BD_BLOCK, 3) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
+__global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     const DevScene& sc = a.sc;
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};

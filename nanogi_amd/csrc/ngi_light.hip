@@ -10,11 +10,18 @@
 namespace ngi {
 
 #define NGI_LT_BLOCK 256
+// Minimum waves per SIMD for the light kernels' register allocation: 5
+// (96 VGPRs, some spilled) measured fastest for ltdirect against 3 (the
+// unconstrained 141 VGPRs) and 4: +22% on the sphere field, +19% on the nave,
+// -4% on the 36-triangle Cornell box.
+#ifndef NGI_LT_WAVES
+#define NGI_LT_WAVES 5
+#endif
 
 // ltdirect in FP32 sends its sensor connections through the wavefront's
 // shadow queue and traces them 64 at a time.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
-__global__ void __launch_bounds__(NGI_LT_BLOCK) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+__global__ void __launch_bounds__(NGI_LT_BLOCK, NGI_LT_WAVES) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
