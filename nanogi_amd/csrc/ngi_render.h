@@ -1,4 +1,5 @@
-// Render session shared by the megakernel and wavefront engines.
+// Render session and the launch entry points of its engines (megakernel,
+// wavefront, persistent path tracer, light tracers, bdpt).
 #pragma once
 
 #include <algorithm>
