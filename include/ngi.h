@@ -237,7 +237,8 @@ typedef struct ngi_render_params {
 } ngi_render_params;
 
 /* Reference defaults: pt, 1280x720, 10,000,000 samples uniform over the
- * image, -m -1, RR (1, 0.5) (src/nanogi.cpp:2000-2019, 583-591). */
+ * image, -m -1, RR (1, 0.5) (src/nanogi.cpp:2000-2019, 583-591); FP32
+ * shading with an FP64 film on the persistent engine. */
 void ngi_render_params_default(ngi_render_params* p);
 
 typedef struct ngi_stats {

@@ -571,7 +571,7 @@ void ngi_render_params_default(ngi_render_params* p) {
     p->seed = 1;
     p->stream_id = 0;
     p->precision = NGI_PRECISION_FP32_FP64ACC;
-    p->engine = NGI_ENGINE_WAVEFRONT;
+    p->engine = NGI_ENGINE_PERSISTENT;     // the fastest engine on every measured workload
 }
 
 }  // extern "C"

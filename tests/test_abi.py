@@ -67,7 +67,7 @@ def test_default_params_are_reference_defaults():
     assert (p.width, p.height, p.num_samples, p.max_num_vertices) == (1280, 720, 10000000, -1)
     assert p.renderer == ngi.RENDERER_PT and p.sampling == ngi.SAMPLING_REFERENCE_UNIFORM
     assert (p.rr_depth, p.rr_prob) == (1, 0.5)
-    assert p.engine == ngi.ENGINE_WAVEFRONT
+    assert p.engine == ngi.ENGINE_PERSISTENT and p.precision == ngi.PRECISION_FP32_FP64ACC
 
 
 def test_unsupported_renderers_fail_before_touching_the_device():
