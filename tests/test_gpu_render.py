@@ -253,3 +253,20 @@ def test_reference_sampling_with_few_paths(renderer):
         err = np.abs(g - o).max(axis=2)
         bad = err > 2e-4 * (np.abs(o).max(axis=2) + 1e-3 * max(np.abs(o).max(), 1e-30))
         assert bad.sum() <= max(1, n // 200), (n, int(bad.sum()))
+
+
+def test_full_size_sphere_field_fp32_against_fp64_shading():
+    """The benchmark's scene at its full size (1,833,314 triangles, 1024x1024,
+    up to 33 vertices, roulette from 4), 4 spp: FP32 shading with the shadow
+    queue against FP64 shading on the same paths.  Size-independent bar: the
+    usual per-pixel agreement and image means within 1e-3."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, 8)
+    info = s.build(ngi.BVH_8)
+    assert s.num_triangles == 1833314 and info["max_depth"] >= 6
+    out = {}
+    for prec in (ngi.PRECISION_FP32_FP64ACC, ngi.PRECISION_FP64_ALL):
+        p = ngi.default_params(renderer="ptdirect", width=1024, height=1024, sampling=ngi.SAMPLING_PER_PIXEL, spp=4,
+                               max_num_vertices=33, rr_depth=4, seed=21, precision=prec)
+        out[prec], st = ngi.render(s, p)
+        assert st["samples"] == 1024 * 1024 * 4 and st["shadow_rays"] > 0
+    _crn_close(out[ngi.PRECISION_FP32_FP64ACC], out[ngi.PRECISION_FP64_ALL], 2e-4, frac=0.97, mean_rtol=1e-3)
