@@ -347,6 +347,41 @@ def bench_renderers(a):
     return out
 
 
+def bench_nave(a):
+    """The nave interior (about 77k triangles) with ptdirect at 1024x1024,
+    `--nave-spp` samples per pixel, at most 16 segments, roulette from depth
+    4: one render per engine (persistent, the plain megakernel, wavefront),
+    plus the FP64-shading megakernel on a 16-spp sample."""
+    import nanogi_amd as ngi
+    scene = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
+    scene.build(8)
+    W = H = 1024
+    spp = a.nave_spp
+    out = {"workload": f"ptdirect, nave ({scene.num_triangles} triangles), {W}x{H}, {spp} spp, max 16 segments, "
+                       f"RR from depth 4", "engines": {}}
+    variants = [("persistent", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, spp),
+                ("megakernel", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC, spp),
+                ("wavefront", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC, spp),
+                ("megakernel_fp64", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL, min(spp, 16))]
+    for name, eng, prec, n in variants:
+        p = ngi.default_params(renderer="ptdirect", width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=n,
+                               max_num_vertices=17, rr_depth=4, seed=1, engine=eng, precision=prec)
+        s = ngi.Session(scene, p)
+        s.render(1)
+        s.sync()
+        s.reset()
+        t0 = time.perf_counter()
+        s.render(n)
+        s.sync()
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        s.close()
+        out["engines"][name] = {"spp": n, "seconds": round(dt, 3), "samples_per_s": round(W * H * n / dt, 1),
+                                "rays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / dt, 1)}
+        log(f"nave {name}: {out['engines'][name]}")
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +409,8 @@ def parse():
                     help="Adam steps of the inverse-rendering workload reported beside the main line (0 = skip)")
     ap.add_argument("--autotune-paths", type=int, default=1 << 24,
                     help="paths per candidate of the single-scene launch tuner on this workload (0 = skip)")
+    ap.add_argument("--nave-spp", type=int, default=256,
+                    help="spp of the nave (ptdirect, 1024^2) engine comparison (0 = skip)")
     ap.add_argument("--renderers", type=int, default=1, help="time every renderer on the Cornell box (0 = skip)")
     ap.add_argument("--rmse-ref-spp", type=int, default=0,
                     help="render an FP64 reference at this spp and report each variant's RMSE against it (0 = skip)")
@@ -574,6 +611,7 @@ def main():
     if a.rmse_ref_spp > 0:
         study = rmse_study(a, scene, params, ngi)
     renderers = bench_renderers(a) if a.renderers else None
+    nave = bench_nave(a) if a.nave_spp > 0 else None
     batch = None
     if a.batch_scenes > 0:
         batch = bench_batch(a)
@@ -604,6 +642,7 @@ def main():
         "adjoint": adjoint,
         "rmse_study": study,
         "renderers": renderers,
+        "nave": nave,
     }
     print(json.dumps(out), flush=True)
     return 0
