@@ -40,16 +40,35 @@ struct WavefrontState {
     uint32_t* host_counts = nullptr;   // pinned mirror
 };
 
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    const int leader = __ffsll((long long)mask) - 1;
+// Queue slots for two queues at once, aggregated per block: each wavefront
+// takes its offset within the block with one LDS atomic per queue, then one
+// thread adds the block's totals to the global counters (one global atomic
+// per block and queue instead of one per wavefront: every wavefront of the
+// grid would otherwise hit the same two addresses).  Every thread of the
+// block must call it.  Returns 0xFFFFFFFF for a lane without the predicate.
+__device__ __forceinline__ void block_push2(uint32_t* counter_a, bool pa, uint32_t& slot_a, uint32_t* counter_b, bool pb,
+                                            uint32_t& slot_b) {
+    __shared__ uint32_t tot[2], base[2];
     const uint32_t lane = __lane_id();
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    return pred ? base + rank : 0xFFFFFFFFu;
+    const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
+    if (threadIdx.x == 0) tot[0] = tot[1] = 0;
+    __syncthreads();
+    uint32_t wa = 0, wb = 0;
+    if (lane == 0) {
+        if (ma) wa = atomicAdd(&tot[0], (uint32_t)__popcll(ma));
+        if (mb) wb = atomicAdd(&tot[1], (uint32_t)__popcll(mb));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        base[0] = tot[0] ? atomicAdd(counter_a, tot[0]) : 0u;
+        base[1] = tot[1] && counter_b ? atomicAdd(counter_b, tot[1]) : 0u;
+    }
+    __syncthreads();
+    wa = __shfl(wa, 0);
+    wb = __shfl(wb, 0);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    slot_a = pa ? base[0] + wa + (uint32_t)__popcll(ma & below) : 0xFFFFFFFFu;
+    slot_b = pb ? base[1] + wb + (uint32_t)__popcll(mb & below) : 0xFFFFFFFFu;
 }
 
 __device__ __forceinline__ void wave_count(unsigned long long* p, uint32_t v) {
@@ -102,7 +121,8 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_generate(RenderArgs a, uint64
         emit = !is_zero(C);
     }
     if (DIRECT) {
-        const uint32_t slot = wave_push(counts + 1, emit);
+        uint32_t slot = 0, unused = 0;
+        block_push2(counts + 1, emit, slot, nullptr, false, unused);
         if (emit) store_shadow(shadow, slot, sr, target, C);
     }
     if (dead) store_path(ray, st, slot_base + i, cam.eye, f3(0.0f, 0.0f, 1.0f), f3(0.0f), 0, 0, 0, 1, sc.sensor_prim, NGI_PRIM_E, -1.0f);
@@ -189,11 +209,9 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
             }
         }
     }
-    if (DIRECT) {
-        const uint32_t slot = wave_push(counts + 1, emit);
-        if (emit) store_shadow(shadow, slot, sr, target, C);
-    }
-    const uint32_t slot = wave_push(counts + 0, alive);
+    uint32_t slot = 0, sslot = 0;
+    block_push2(counts + 0, alive, slot, DIRECT ? counts + 1 : nullptr, DIRECT && emit, sslot);
+    if (DIRECT && emit) store_shadow(shadow, sslot, sr, target, C);
     if (alive) store_path(ray_out, st_out, slot, ns.p, wo, T, pixel, c0, c1, nv, hp, type);
 }
 
