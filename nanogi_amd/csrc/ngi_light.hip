@@ -1,6 +1,6 @@
 // Light tracers (lt, ltdirect) on the persistent engine: lanes take the
-// next light-path id when their path ends (one atomic per wavefront), as the
-// path tracers' persistent kernel does.  The vertex loop is in ngi_light.h.
+// next light-path id when their path ends (from the wavefront's chunk of
+// ids), as the path tracers' persistent kernel does.  The vertex loop is in ngi_light.h.
 #include <algorithm>
 #include <type_traits>
 
@@ -20,7 +20,7 @@ namespace ngi {
 
 // ltdirect in FP32 sends its sensor connections through the wavefront's
 // shadow queue and traces them 64 at a time.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool CHUNKED>
 __global__ void __launch_bounds__(NGI_LT_BLOCK, NGI_LT_WAVES) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
@@ -32,22 +32,17 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, NGI_LT_WAVES) light_kernel(Rende
     const uint32_t wave = blockIdx.x * (NGI_LT_BLOCK / 64) + (threadIdx.x >> 6);
     uint32_t qhead = 0, qcount = 0;   // wave-uniform shadow queue state (< NGI_SHQ)
     LightState<R> ls;
-    bool active = false, exhausted = false;
+    PathFeed feed;
+    bool active = false;
     for (;;) {
+        const bool exhausted = feed.done();
         if (!exhausted) {
             const unsigned long long need = __ballot(!active);
             if (need) {
-                const int leader = __ffsll((long long)need) - 1;
-                unsigned long long base = 0;
-                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
-                base = __shfl(base, leader);
-                if (base + __popcll(need) >= a.path_count) exhausted = true;
-                if (!active) {
-                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
-                    if (local < a.path_count) {
-                        cnt[CNT_SAMPLES]++;
-                        active = light_start<R>(a, a.path_begin + local, ls);
-                    }
+                const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
+                if (!active && id != ~0ull) {
+                    cnt[CNT_SAMPLES]++;
+                    active = light_start<R>(a, a.path_begin + id, ls);
                 }
             }
         }
@@ -76,7 +71,7 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, NGI_LT_WAVES) light_kernel(Rende
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4 : 0;
-    auto kern = light_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
+    auto kern = a.path_chunk ? light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>;
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_LT_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;

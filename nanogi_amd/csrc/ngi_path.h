@@ -37,6 +37,7 @@ struct RenderArgs {
     double rr_prob_d;
     uint32_t seed, stream_id;
     unsigned long long* counters;   // CNT_N entries or null
+    uint32_t path_chunk;            // persistent engines: path ids per counter atomic (0 = exactly the ids needed)
 };
 
 template <typename R> __device__ __forceinline__ R unif(uint32_t x);
@@ -236,6 +237,64 @@ __device__ __forceinline__ V3<R> camera_sample(const RenderArgs& a, const Cam<R>
     }
     return camera_dir(cam, rx, ry);
 }
+
+// Path ids of a persistent wavefront, taken from the launch's counter.
+// Every wavefront of the grid shares that counter.  With chunk = 0 a refill
+// takes exactly the ids its idle lanes need, which keeps the lanes of a
+// wavefront on paths from all over the current frontier; with chunk > 0 it
+// takes at least `chunk` ids and hands out the rest on later refills, one
+// atomic per chunk.  The state is wave-uniform; take() must be called by
+// every lane of the wave.
+struct PathFeed {
+    unsigned long long nxt = 0, end = 0;   // reserved ids not handed out yet: [nxt, end)
+    bool drained = false;                  // the counter has passed the path count
+
+    __device__ __forceinline__ bool done() const { return drained && nxt >= end; }
+
+    // Consecutive ids for the lanes set in `need` (rank order): this lane's
+    // id, or ~0 when the ids ran out.  CHUNKED = false ignores `chunk`.
+    template <bool CHUNKED>
+    __device__ __forceinline__ unsigned long long take(unsigned long long* counter, unsigned long long count, unsigned long long need,
+                                                       uint32_t chunk) {
+        const uint32_t lane = __lane_id();
+        const unsigned long long k = (unsigned long long)__popcll(need);
+        if (!CHUNKED) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long b = 0;
+            if ((int)lane == leader) b = atomicAdd(counter, k);
+            b = __shfl(b, leader);
+            if (b + k >= count) drained = true;
+            const unsigned long long id = b + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+            return ((need >> lane) & 1ull) && id < count ? id : ~0ull;
+        }
+        const unsigned long long left = end - nxt;
+        unsigned long long nb = 0, ne = 0;
+        if (left < k && !drained) {
+            const unsigned long long ask = k - left > chunk ? k - left : chunk;
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long b = 0;
+            if ((int)lane == leader) b = atomicAdd(counter, ask);
+            b = __shfl(b, leader);
+            if (b < count) nb = b, ne = b + ask < count ? b + ask : count;
+            if (b + ask >= count) drained = true;
+        }
+        unsigned long long id = ~0ull;
+        if ((need >> lane) & 1ull) {
+            const unsigned long long r = (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+            if (r < left) id = nxt + r;
+            else if (r - left < ne - nb) id = nb + (r - left);
+        }
+        if (k <= left) {
+            nxt += k;
+        } else if (ne > nb) {
+            const unsigned long long used = k - left < ne - nb ? k - left : ne - nb;
+            nxt = nb + used, end = ne;
+        } else {
+            nxt = end;
+        }
+        return id;
+    }
+};
 
 // Per-lane path state between vertices of ProcessSample_PT/_PTDirect.
 template <typename R>

@@ -19,6 +19,11 @@ struct TimedLaunch {
 
 struct WavefrontState;   // ngi_wavefront.hip
 
+// Scenes below this many triangles take path ids in chunks (RenderArgs::path_chunk).
+#ifndef NGI_CHUNKED_FEED_TRIS
+#define NGI_CHUNKED_FEED_TRIS 16384
+#endif
+
 // Concurrency of a persistent launch.  want > 0 caps the paths in flight
 // (rounded up to whole blocks); 0 keeps every block the device can hold
 // resident; want < 0 launches nothing and only reports that full size.

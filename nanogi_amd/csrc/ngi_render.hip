@@ -59,11 +59,11 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
 }
 
 // Persistent lanes with path regeneration: a lane whose path ended takes the
-// next path id (one atomic per wavefront and iteration), so every iteration
+// next path id (from its wavefront's chunk of ids), so every iteration
 // traces with all lanes busy regardless of how uneven path lengths are.
 // With ptdirect in FP32 the NEE shadow rays go through the wavefront's
 // shadow queue (q, a ring per wavefront) and are traced 64 at a time.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool CHUNKED>
 __global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];
@@ -75,23 +75,17 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(
     const uint32_t wave = blockIdx.x * (NGI_MK_BLOCK / 64) + (threadIdx.x >> 6);
     uint32_t qhead = 0, qcount = 0;   // wave-uniform: oldest slot, queued rays (< NGI_SHQ)
     PathState<R> ps;
-    bool active = false, exhausted = false;
+    PathFeed feed;
+    bool active = false;
     for (;;) {
-        if (!exhausted) {
+        if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
             if (need) {
-                const int leader = __ffsll((long long)need) - 1;
-                unsigned long long base = 0;
-                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
-                base = __shfl(base, leader);
-                if (base + __popcll(need) >= a.path_count) exhausted = true;
-                if (!active) {
-                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
-                    if (local < a.path_count) {
-                        path_start<R>(a, a.path_begin + local, cam, ps);
-                        cnt[CNT_SAMPLES]++;
-                        active = true;
-                    }
+                const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
+                if (!active && id != ~0ull) {
+                    path_start<R>(a, a.path_begin + id, cam, ps);
+                    cnt[CNT_SAMPLES]++;
+                    active = true;
                 }
             }
         }
@@ -128,7 +122,7 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
-    auto kern = persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT>;
+    auto kern = a.path_chunk ? persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>;
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_MK_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;
@@ -305,6 +299,12 @@ RenderArgs Session::args(void* target_film) const {
     a.seed = p.seed;
     a.stream_id = p.stream_id;
     a.counters = reinterpret_cast<unsigned long long*>(counters);
+    // Persistent engines: on small scenes an iteration is short and one
+    // counter atomic per refill serialises the grid (ptdirect / ltdirect on the
+    // 36-triangle Cornell box: +27% / +40% with chunks of 128); on large ones
+    // exact refills keep each wavefront's paths spread over the frontier
+    // (chunks cost 5-10% on the 77k-triangle nave and the sphere field).
+    a.path_chunk = scene->dev->view.num_tris < NGI_CHUNKED_FEED_TRIS ? 128u : 0u;
     return a;
 }
 
