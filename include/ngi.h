@@ -289,8 +289,8 @@ void ngi_session_destroy(ngi_session* s);
 /* the launch autotuner).  Renders tune_paths paths (0 = 2^24) of the   */
 /* workload *p describes with each candidate -- persistent lanes at    */
 /* the full, 3/4 and 1/2 device fill; for pt/ptdirect also wavefront   */
-/* queues of 2^21, 2^22 and 2^23 paths (FP32 shading) and the          */
-/* megakernel -- as one warm-up and two timed renders (wall time, the  */
+/* queues of 2^22 to 2^25 paths (FP32 shading) and the megakernel --   */
+/* as one warm-up and two timed renders (wall time, the                */
 /* better one counts), then writes the fastest into p->engine and      */
 /* p->paths_in_flight.  Every candidate draws the same random numbers, */
 /* so the choice changes speed, not the estimate.  trials (optional,   */

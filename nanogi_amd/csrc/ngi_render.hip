@@ -546,7 +546,8 @@ int ngi_autotune(ngi_scene* scene, ngi_render_params* p, int64_t tune_paths, ngi
     for (int64_t l : {(int64_t)0, full * 3 / 4, full / 2}) cands.push_back({NGI_ENGINE_PERSISTENT, l, 0.0});
     if (!persistent_only(q.renderer)) {
         if (q.precision != NGI_PRECISION_FP64_ALL)
-            for (int64_t c : {(int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) cands.push_back({NGI_ENGINE_WAVEFRONT, c, 0.0});
+            for (int64_t c : {(int64_t)1 << 22, (int64_t)1 << 23, (int64_t)1 << 24, (int64_t)1 << 25})
+                cands.push_back({NGI_ENGINE_WAVEFRONT, c, 0.0});
         cands.push_back({NGI_ENGINE_MEGAKERNEL, 0, 0.0});
     }
     // Each candidate: one untimed warm-up render, then the best of two.

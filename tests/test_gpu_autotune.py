@@ -47,7 +47,7 @@ def test_autotune_picks_its_fastest_candidate(renderer):
     if renderer == "bdpt":
         assert len(trials) == 3 and engines == {ngi.ENGINE_PERSISTENT}
     else:
-        assert len(trials) == 7 and engines == {ngi.ENGINE_PERSISTENT, ngi.ENGINE_WAVEFRONT, ngi.ENGINE_MEGAKERNEL}
+        assert len(trials) == 8 and engines == {ngi.ENGINE_PERSISTENT, ngi.ENGINE_WAVEFRONT, ngi.ENGINE_MEGAKERNEL}
     assert all(t["ms"] > 0 and t["samples_per_s"] > 0 for t in trials)
     best = min(trials, key=lambda t: t["ms"])
     assert (tuned.engine, tuned.paths_in_flight) == (best["engine"], best["paths_in_flight"])
@@ -77,7 +77,7 @@ def test_autotune_reference_uniform_sampling():
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
     p = ngi.default_params(renderer="pt", width=32, height=24, num_samples=50000, max_num_vertices=6)
     tuned, trials = ngi.autotune(s, p, tune_paths=1 << 16)
-    assert tuned.num_samples == 50000 and len(trials) == 7
+    assert tuned.num_samples == 50000 and len(trials) == 8
     f, st = ngi.render(s, tuned)
     assert st["samples"] == 50000 and np.isfinite(f).all() and f.mean() > 0
 

@@ -186,7 +186,7 @@ def test_cli_autotune(tmp_path):
     r = _cli("-r", "ptdirect", "-i", CORNELL, "-o", str(out), "-w", "40", "-h", "30", "--spp", "16", "-m", "6",
              "--seed", "3", "--autotune", "100000")
     assert r.returncode == 0, r.stderr
-    assert r.stdout.count("Autotune:") == 8 and "# of samples: 19200" in r.stdout
+    assert r.stdout.count("Autotune:") == 9 and "# of samples: 19200" in r.stdout
     img = _read_pfm(out)
     assert img.shape == (30, 40, 3) and np.isfinite(img).all() and img.mean() > 0
 
