@@ -326,14 +326,19 @@ static int wf_alloc(WavefrontState& W, uint32_t cap) {
 int Session::render_wavefront(void* target, uint64_t total) {
     if (p.precision == NGI_PRECISION_FP64_ALL)
         return fail(NGI_ERR_UNSUPPORTED, "the wavefront engine shades in FP32; FP64 shading runs on the megakernel");
+    // Default capacity: the render's paths rounded up to a power of two,
+    // between 2^16 and 2^25 (8 GB of queues); the queues grow when a later
+    // render is larger.  Larger queues mean fewer iterations: 1,033 / 1,148 /
+    // 1,216 / 1,245 M samples/s for 2^22 / 2^23 / 2^24 / 2^25 on the
+    // 1.8M-triangle sphere field.
+    uint32_t def = 1u << 16;
+    while (def < (1u << 25) && def < total) def <<= 1;
+    if (wf && p.paths_in_flight <= 0 && def > wf->capacity) {
+        NGI_HIP_CHECK(hipStreamSynchronize(stream()));
+        release_engine();
+    }
     if (!wf) {
         wf = new WavefrontState();
-        // Default capacity: the first render's paths rounded up to a power of
-        // two, between 2^16 and 2^25 (8 GB of queues).  Larger queues mean
-        // fewer iterations: 1,033 / 1,148 / 1,216 / 1,245 M samples/s for
-        // 2^22 / 2^23 / 2^24 / 2^25 on the 1.8M-triangle sphere field.
-        uint32_t def = 1u << 16;
-        while (def < (1u << 25) && def < total) def <<= 1;
         uint32_t cap = p.paths_in_flight > 0 ? (uint32_t)p.paths_in_flight : def;
         cap = std::max<uint32_t>(cap, 256);
         int st = wf_alloc(*wf, cap);

@@ -100,3 +100,24 @@ def test_scratch_grows_between_queued_launches(renderer):
     b.close()
     assert np.abs(fb).max() > 0
     _same(fa, fb)
+
+
+def test_wavefront_default_queues_grow_with_the_render():
+    """With paths_in_flight = 0 the wavefront queues are sized from the render
+    (2^16 paths for the first, 16,384-path render here) and reallocated for
+    a larger later one (245,760 paths -> 2^18); the film equals one render
+    of all the samples."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    p = _p("ptdirect", 128, 128, 1, 6)
+    p.engine = ngi.ENGINE_WAVEFRONT
+    a = ngi.Session(s, p)
+    a.render(1)
+    a.render(15)
+    fa = a.film()
+    a.close()
+    b = ngi.Session(s, p)
+    b.render(16)
+    fb = b.film()
+    b.close()
+    assert np.abs(fb).max() > 0
+    _same(fa, fb)
