@@ -494,13 +494,14 @@ def main():
     sess.sync()
     elapsed = time.perf_counter() - t0
     st = sess.stats()
-    film, var = sess.film(variance=True)
+    # The batch variance needs two batches (steps).
+    var = sess.film(variance=True)[1] if a.steps >= 2 else None
     samples = a.steps * a.spp_per_step * npix
     value = samples / elapsed
     rays = st["extension_rays"] + st["shadow_rays"]
     spp_done = a.steps * a.spp_per_step
-    rmse_now = float(np.sqrt(np.mean(var))) if a.steps >= 2 else float("nan")
-    rmse_1024 = rmse_now * math.sqrt(spp_done / 1024.0)
+    rmse_now = float(np.sqrt(np.mean(var))) if var is not None else None
+    rmse_1024 = rmse_now * math.sqrt(spp_done / 1024.0) if rmse_now is not None else None
     log(f"timed: {samples} paths in {elapsed:.3f} s -> {value / 1e6:.2f} M samples/s, {rays / elapsed / 1e9:.3f} G rays/s")
 
     # Roofline of the dominant kernel.

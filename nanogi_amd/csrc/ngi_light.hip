@@ -17,11 +17,17 @@ namespace ngi {
 #ifndef NGI_LT_WAVES
 #define NGI_LT_WAVES 5
 #endif
+// ltdirect's variant for small scenes (chunked path feed) gets 4, without
+// spills: 1,072 against 1,056 M samples/s on the Cornell box (lt loses 3%
+// with 4 and keeps 5).
+#ifndef NGI_LT_WAVES_SMALL
+#define NGI_LT_WAVES_SMALL 4
+#endif
 
 // ltdirect in FP32 sends its sensor connections through the wavefront's
 // shadow queue and traces them 64 at a time.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool CHUNKED>
-__global__ void __launch_bounds__(NGI_LT_BLOCK, NGI_LT_WAVES) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+__global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES) light_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};

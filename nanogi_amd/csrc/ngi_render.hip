@@ -20,9 +20,15 @@ namespace ngi {
 // Minimum waves per SIMD the register allocator must leave room for in the
 // persistent kernel: 5 (at most 102 VGPRs) measured fastest on the 1.8M
 // triangle ptdirect workload (960 vs 846 M samples/s unconstrained at 147
-// VGPRs, 933 at 4 waves, 948 at 6).
+// VGPRs, 933 at 4 waves, 948 at 6; with the shadow queue 1,447 M at 5, 1,362
+// at 4, 1,426 at 6).  ptdirect's variant for small scenes (chunked path
+// feed) gets 4, without spills: on the 36-triangle Cornell box 1,200 M
+// samples/s against 1,075 M with 5 (pt loses 1% with 4 and keeps 5).
 #ifndef NGI_PK_WAVES
 #define NGI_PK_WAVES 5
+#endif
+#ifndef NGI_PK_WAVES_SMALL
+#define NGI_PK_WAVES_SMALL 4
 #endif
 
 template <typename T>
@@ -64,7 +70,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK) megakernel(RenderArgs a) {
 // With ptdirect in FP32 the NEE shadow rays go through the wavefront's
 // shadow queue (q, a ring per wavefront) and are traced 64 at a time.
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool CHUNKED>
-__global__ void __launch_bounds__(NGI_MK_BLOCK, NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+__global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES) persistent_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     constexpr bool QUEUE = DIRECT && std::is_same<R, float>::value;
     extern __shared__ uint32_t stk[];
     uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
