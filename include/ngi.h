@@ -230,7 +230,8 @@ typedef struct ngi_render_params {
     int precision;           /* NGI_PRECISION_* */
     int engine;              /* NGI_ENGINE_* */
     int paths_in_flight;     /* concurrent paths: wavefront queue capacity, or lanes of the
-                              * persistent path-tracing kernel (0 = auto) */
+                              * persistent kernels (0 = auto: full device fill; wavefront
+                              * queues sized to the render, 2^16..2^25 paths, growing) */
     int block_size;          /* threads per block (0 = auto) */
     int collect_counters;    /* count node visits / triangle tests (slower) */
     int time_kernels;        /* record per-kernel HIP-event times */
