@@ -170,6 +170,50 @@ def test_beckmann_sampling_matches_pdf(alpha, theta_i):
     assert abs(acc / n - tot / n) < 0.03, (acc / n, tot / n)
 
 
+def _hemisphere_pairs(rng, n):
+    v = rng.normal(size=(n, 2, 3))
+    v[..., 2] = np.abs(v[..., 2])
+    return v / np.linalg.norm(v, axis=2, keepdims=True)
+
+
+@pytest.mark.parametrize("alpha", [0.1, 0.4])
+def test_glossy_brdf_is_reciprocal(alpha):
+    """f(wi, wo) = f(wo, wi) for the Beckmann conductor (rt.hpp:1032-1047;
+    both cosines in the denominator, F and G of the half vector)."""
+    prim = _glossy(alpha)
+    for wi, wo in _hemisphere_pairs(np.random.default_rng(2), 50):
+        _, _, f1 = oracle.bsdf(prim, ngi.PRIM_G, wi, wo=wo)
+        _, _, f2 = oracle.bsdf(prim, ngi.PRIM_G, wo, wo=wi)
+        np.testing.assert_allclose(f1, f2, rtol=1e-12, atol=1e-200)   # far-off-peak values are subnormal
+
+
+@pytest.mark.parametrize("alpha,theta_i", [(0.3, 0.0), (0.3, 50.0), (0.6, 30.0)])
+def test_glossy_albedo_by_bsdf_and_cosine_sampling(alpha, theta_i):
+    """The albedo (integral of f over projected solid angle) estimated two
+    ways: with the BSDF's own sampler (weight f / pdf, rejected samples 0)
+    and with cosine-weighted directions (weight pi f).  A pdf that is off by
+    a cosine or a Jacobian, or a D whose normalisation differs between f and
+    pdf, moves one estimate and not the other."""
+    prim = _glossy(alpha)
+    ti = np.radians(theta_i)
+    wi = (np.sin(ti), 0.0, np.cos(ti))
+    rng = np.random.default_rng(11)
+    n = 20000
+    a = np.zeros(n)
+    for i, u in enumerate(rng.random((n, 3))):
+        wo, pdf, fs = oracle.bsdf(prim, ngi.PRIM_G, wi, u=u)
+        if wo[2] > 0 and pdf > 0:
+            a[i] = fs[0] / pdf
+    b = np.zeros(n)
+    for i, u in enumerate(rng.random((n, 2))):
+        r, ph = np.sqrt(u[0]), 2 * np.pi * u[1]
+        _, _, fs = oracle.bsdf(prim, ngi.PRIM_G, wi, wo=(r * np.cos(ph), r * np.sin(ph), np.sqrt(max(0.0, 1 - u[0]))))
+        b[i] = np.pi * fs[0]
+    se = np.sqrt(a.var() / n + b.var() / n)
+    assert 0.5 < a.mean() < 1.0
+    assert abs(a.mean() - b.mean()) < 4 * se, (a.mean(), b.mean(), se)
+
+
 def test_lambert_sampling_is_cosine_weighted():
     rng = np.random.default_rng(3)
     prim = ngi.make_primitive(type=ngi.PRIM_D, D_R=(0.5, 0.5, 0.5))
