@@ -118,6 +118,24 @@ class LoadedScene:
                                                film.ctypes.data, int(num_threads), counts.ctypes.data))
         return film.reshape(params.height, params.width, 3), counts
 
+    def camera(self, u0, u1):
+        """(wo, (rx, ry), We, pdf) of the pinhole for raster position (u0, u1)."""
+        L = lib()
+        L.oracle_scene_camera.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L.oracle_scene_camera.restype = ctypes.c_int
+        out = np.zeros(7, dtype=np.float64)
+        _check(L.oracle_scene_camera(self.h, float(u0), float(u1), out.ctypes.data))
+        return out[0:3], out[3:5], out[5], out[6]
+
+    def light(self, k, ux, uy):
+        """(p, gn, position pdf, selection pdf) of light k sampled at (ux, uy)."""
+        L = lib()
+        L.oracle_scene_light.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L.oracle_scene_light.restype = ctypes.c_int
+        out = np.zeros(8, dtype=np.float64)
+        _check(L.oracle_scene_light(self.h, int(k), float(ux), float(uy), out.ctypes.data))
+        return out[0:3], out[3:6], out[6], out[7]
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().oracle_scene_free(self.h)

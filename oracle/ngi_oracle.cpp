@@ -1203,6 +1203,42 @@ void* oracle_scene_new(const ngi_scene_desc* d) {
 
 void oracle_scene_free(void* h) { delete reinterpret_cast<Scene*>(h); }
 
+// Pinhole sensor of a loaded scene: raster (u0, u1) -> SampleDirection
+// (rt.hpp:733-740) -> out = {wo.xyz, RasterPosition rx, ry (rt.hpp:1348-1378),
+// We (EvaluateDirection, rt.hpp:955-978), pdf (EvaluateDirectionPDF,
+// rt.hpp:1189-1212)}.
+int oracle_scene_camera(void* h, double u0, double u1, double out[7]) {
+    const Scene& s = *reinterpret_cast<Scene*>(h);
+    if (s.sensor < 0 || s.prims[s.sensor].p.sensor_type != NGI_SENSOR_PINHOLE) return NGI_ERR_UNSUPPORTED;
+    const Prim& P = s.prims[s.sensor];
+    Geom g;
+    g.degenerated = true;
+    g.p = P.eye;
+    V wo;
+    SampleDirection(P, NGI_PRIM_E, g, V(), u0, u1, 0.0, wo);
+    double rx = -1.0, ry = -1.0;
+    RasterPosition(P, wo, rx, ry);
+    const V W = EvaluateDirection(P, NGI_PRIM_E, g, V(), wo, false);
+    out[0] = wo.x, out[1] = wo.y, out[2] = wo.z, out[3] = rx, out[4] = ry, out[5] = W.x;
+    out[6] = EvaluateDirectionPDF(P, NGI_PRIM_E, g, V(), wo);
+    return NGI_OK;
+}
+
+// Light k (in scene order) of a loaded scene: SamplePosition (rt.hpp:483-562)
+// at (ux, uy) -> out = {p.xyz, gn.xyz, EvaluatePositionPDF (rt.hpp:645-662),
+// emitter selection pdf (rt.hpp:2338-2352)}.
+int oracle_scene_light(void* h, int k, double ux, double uy, double out[8]) {
+    const Scene& s = *reinterpret_cast<Scene*>(h);
+    if (k < 0 || k >= (int)s.lights.size()) return NGI_ERR_INVALID_ARG;
+    const Prim& P = s.prims[s.lights[k]];
+    Geom g;
+    SamplePosition(s, P, ux, uy, g);
+    out[0] = g.p.x, out[1] = g.p.y, out[2] = g.p.z, out[3] = g.gn.x, out[4] = g.gn.y, out[5] = g.gn.z;
+    out[6] = EvaluatePositionPDF(P);
+    out[7] = EmitterPDF(s, P);
+    return NGI_OK;
+}
+
 // As oracle_render_paths, on a loaded scene: only the sample loop is timed
 // by callers (the region the reference times, src/nanogi.cpp:201-216).
 int oracle_scene_render_paths(void* h, const ngi_render_params* prm, uint64_t path_begin, uint64_t path_count, double* film,

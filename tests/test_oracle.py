@@ -425,3 +425,65 @@ def test_bdpt_equals_ptdirect(scene):
     se = np.hypot(np.std(pt, 0, ddof=1), np.std(bd, 0, ddof=1)) / 2
     assert np.mean(pt) > 0
     assert np.all(np.abs(d) <= 4.5 * se + 1e-3 * np.mean(pt)), (d, se)
+
+
+# ---------------------------------------------------------------------------
+# Pinhole sensor and light sampling
+# ---------------------------------------------------------------------------
+def test_pinhole_raster_round_trip_and_importance_over_pdf():
+    """RasterPosition(SampleDirection(u)) = u, We / pdf = 1 (rt.hpp:733-740,
+    955-978, 1189-1212, 1348-1378); the vertical field of view is `fov`, the
+    horizontal one follows the aspect W/H, and raster row 0 is the bottom."""
+    B = scenes.SceneBuilder()
+    p = B.prim(type=ngi.PRIM_D, D_R=(0.5, 0.5, 0.5))
+    B.tri(p, (-1, -1, 0), (1, -1, 0), (0, 1, 0))
+    eye, center, fov, aspect = np.array([1.0, 2.0, 3.0]), np.array([0.0, 0.2, 0.0]), 50.0, 1.5
+    B.camera(tuple(eye), tuple(center), up=(0, 1, 0), fov=fov)
+    ls = oracle.LoadedScene(B.scene(aspect=aspect))
+    rng = np.random.default_rng(4)
+    for u0, u1 in rng.random((200, 2)):
+        wo, (rx, ry), We, pdf = ls.camera(u0, u1)
+        assert np.linalg.norm(wo) == pytest.approx(1.0, abs=1e-12)
+        assert rx == pytest.approx(u0, abs=1e-12) and ry == pytest.approx(u1, abs=1e-12)
+        assert We == pytest.approx(pdf, rel=1e-12) and pdf > 0
+    axis = (center - eye) / np.linalg.norm(center - eye)
+    wo, _, _, _ = ls.camera(0.5, 0.5)
+    np.testing.assert_allclose(wo, axis, atol=1e-12)
+    up, _, _, _ = ls.camera(0.5, 1.0)
+    right, _, _, _ = ls.camera(1.0, 0.5)
+    assert np.degrees(np.arccos(np.dot(up, axis))) == pytest.approx(fov / 2, abs=1e-9)
+    assert np.arctan(np.tan(np.arccos(np.dot(right, axis)))) == pytest.approx(
+        np.arctan(aspect * np.tan(np.radians(fov / 2))), abs=1e-12)
+    bottom, _, _, _ = ls.camera(0.5, 0.0)
+    assert bottom[1] < axis[1]   # row 0 looks below the axis (y up)
+
+
+def test_area_light_samples_triangles_by_area_and_uniformly():
+    """SampleTriangleMesh (rt.hpp:488-540): a triangle is chosen with
+    probability proportional to its area by the CDF search and the point is
+    uniform in it; the position pdf is 1 / (total area) (rt.hpp:649-652) and
+    the emitter is chosen uniformly among the lights (rt.hpp:2321-2352)."""
+    B = scenes.SceneBuilder()
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=(1, 1, 1))
+    B.tri(li, (0, 0, 0), (1, 0, 0), (0, 2, 0))          # area 1
+    B.tri(li, (5, 0, 0), (8, 0, 0), (5, 2, 0))          # area 3
+    other = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=(1, 1, 1))
+    B.tri(other, (0, 0, 5), (1, 0, 5), (0, 1, 5))
+    B.camera((0, 0, 10), (0, 0, 0))
+    ls = oracle.LoadedScene(B.scene())
+    rng = np.random.default_rng(9)
+    n = 20000
+    P = np.zeros((n, 3))
+    for i, (ux, uy) in enumerate(rng.random((n, 2))):
+        P[i], gn, pdf_pos, pdf_sel = ls.light(0, ux, uy)
+        assert pdf_pos == pytest.approx(0.25, rel=1e-12) and pdf_sel == 0.5
+        np.testing.assert_allclose(gn, (0, 0, 1), atol=1e-12)
+    in_a = P[:, 0] < 3.0
+    assert abs(in_a.mean() - 0.25) < 4 * np.sqrt(0.25 * 0.75 / n)
+    for sel, tri in ((in_a, np.array([[0, 0, 0], [1, 0, 0], [0, 2, 0]])), (~in_a, np.array([[5, 0, 0], [8, 0, 0], [5, 2, 0]]))):
+        q = P[sel]
+        # uniform in the triangle: the mean is its centroid, and every point is inside it
+        se = q.std(axis=0) / np.sqrt(len(q)) + 1e-15
+        assert np.all(np.abs(q.mean(axis=0) - tri.mean(axis=0)) < 4 * se)
+        lo, hi = tri.min(axis=0), tri.max(axis=0)
+        assert np.all(q >= lo - 1e-12) and np.all(q <= hi + 1e-12)
