@@ -346,7 +346,8 @@ typedef struct ngi_batch_opts {
     int block_size;      /* 64, 128 or 256 threads; 0 = autotune */
     int blocks_per_cu;   /* resident blocks per CU (clamped to occupancy); 0 = autotune */
     int use_graph;       /* 1 = replay a captured hipGraph, 0 = plain launches */
-    int chunk_paths;     /* paths a wavefront takes from one scene at a time (<= 2^24); 0 = autotune */
+    int chunk_paths;     /* paths a wavefront takes from one scene at a time (<= 2^24); 0 = about
+                          * 16 chunks per resident wave, a power of two in [256, 8192] */
     int64_t tune_paths;  /* paths per autotuner trial, drawn proportionally from every scene */
 } ngi_batch_opts;
 typedef struct ngi_batch ngi_batch;

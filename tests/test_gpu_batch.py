@@ -107,7 +107,7 @@ def test_batch_graph_replays_frames_and_follows_scene_updates():
     b = ngi.Batch(scenes, params)
     cfg = b.config()
     assert cfg["block_size"] in (64, 128, 256) and cfg["blocks_per_cu"] >= 1 and cfg["use_graph"] == 1
-    assert cfg["chunk_paths"] in (256, 2048)
+    assert cfg["chunk_paths"] == 256   # a few thousand paths: the smallest chunk
     for frame in (0, 1, 0, 2):
         b.render(frame)
         b.sync()
@@ -171,3 +171,27 @@ def test_batch_with_an_empty_scene_and_single_path_scenes():
         ref, _ = ngi.render(s, p)
         np.testing.assert_allclose(f, ref, rtol=2e-4, atol=1e-6 * max(np.abs(ref).max(), 1.0))
     assert np.count_nonzero(films[1].sum(axis=2)) == 1
+
+
+def test_batch_chunk_size_follows_the_paths_per_wave():
+    """Without a fixed chunk size a batch takes about 16 chunks per resident
+    wave (a power of two in [256, 8192]): 2 x 512^2 x 128 spp = 67M paths get
+    chunks above 256.  The chunk size changes only which wave renders which
+    paths, so the films equal those of chunks of 256."""
+    scenes = [ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 40 + i, 1.0, n) for i, n in enumerate((200, 3000))]
+    params = [_p("ptdirect", 512, 512, 128, seed=3 + i, stream_id=i) for i in range(2)]
+    auto = ngi.Batch(scenes, params, ngi.batch_opts(block_size=64, use_graph=0))
+    c = auto.config()["chunk_paths"]
+    assert 256 < c <= 8192 and c & (c - 1) == 0
+    auto.render(0)
+    auto.sync()
+    fa = [auto.film(i) for i in range(2)]
+    auto.close()
+    small = ngi.Batch(scenes, params, ngi.batch_opts(block_size=64, use_graph=0, chunk_paths=256))
+    small.render(0)
+    small.sync()
+    for i in range(2):
+        fb = small.film(i)
+        assert np.abs(fb).max() > 0
+        np.testing.assert_allclose(fa[i], fb, rtol=1e-9, atol=1e-12 * np.abs(fb).max())
+    small.close()
