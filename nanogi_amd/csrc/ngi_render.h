@@ -67,6 +67,7 @@ struct LaneMem {
         *bytes = 0;
         if (hipMalloc(ptr, n) != hipSuccess) {
             *ptr = nullptr;
+            (void)hipGetLastError();   // a later call's launch check must not see this refusal
             return fail(NGI_ERR_OOM, "cannot allocate %zu bytes of per-lane records", n);
         }
         *bytes = n;
@@ -108,6 +109,7 @@ struct Session {
     ngi_stats stats;
     std::vector<TimedLaunch> pending;
     WavefrontState* wf = nullptr;
+    uint32_t wf_cap_max = 1u << 25;   // default wavefront queues: largest capacity not refused by the allocator
 
     hipStream_t stream() const { return scene->dev->stream; }
     int init(ngi_scene* sc, const ngi_render_params& pp);

@@ -331,18 +331,23 @@ int Session::render_wavefront(void* target, uint64_t total) {
     // render is larger.  Larger queues mean fewer iterations: 1,033 / 1,148 /
     // 1,216 / 1,245 M samples/s for 2^22 / 2^23 / 2^24 / 2^25 on the
     // 1.8M-triangle sphere field.
+    // When the device cannot hold default queues, they halve (down to 2^16)
+    // until they fit, and later renders stay at the size that fitted.
     uint32_t def = 1u << 16;
-    while (def < (1u << 25) && def < total) def <<= 1;
+    while (def < wf_cap_max && def < total) def <<= 1;
     if (wf && p.paths_in_flight <= 0 && def > wf->capacity) {
         NGI_HIP_CHECK(hipStreamSynchronize(stream()));
         release_engine();
     }
-    if (!wf) {
+    while (!wf) {
         wf = new WavefrontState();
-        uint32_t cap = p.paths_in_flight > 0 ? (uint32_t)p.paths_in_flight : def;
-        cap = std::max<uint32_t>(cap, 256);
-        int st = wf_alloc(*wf, cap);
-        if (st) return st;
+        const uint32_t cap = std::max<uint32_t>(p.paths_in_flight > 0 ? (uint32_t)p.paths_in_flight : def, 256);
+        const int st = wf_alloc(*wf, cap);
+        if (!st) break;
+        release_engine();          // frees what was allocated; wf = nullptr
+        (void)hipGetLastError();   // the refused allocation must not fail the next launch check
+        if (st != NGI_ERR_OOM || p.paths_in_flight > 0 || cap <= (1u << 16)) return st;
+        wf_cap_max = def = cap / 2;
     }
     const bool direct = p.renderer == NGI_RENDERER_PTDIRECT;
     const int width = scene->dev->view.bvh_width;
