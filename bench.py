@@ -623,8 +623,9 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(elapsed * 1e3 / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 film)" if precision == ngi.PRECISION_FP32_FP64ACC
-        else ("f32" if precision == ngi.PRECISION_FP32_ALL else "f64 shading, f32 traversal"),
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == ngi.PRECISION_FP64_ALL else "f32",
+        "precision": {"traversal": "f32", "shading": "f64" if precision == ngi.PRECISION_FP64_ALL else "f32",
+                      "film": "f32" if precision == ngi.PRECISION_FP32_ALL else "f64"},
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": f"ptdirect, displaced icosphere ({a.detail} subdivisions) on 512^2 heightfield, "
                                f"{ntris} triangles, {W}x{H}, max 32 segments, RR from depth 4",
