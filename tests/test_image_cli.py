@@ -201,3 +201,18 @@ def test_cli_reference_sampling_and_progress_images(tmp_path):
     assert out.exists() and (tmp_path / "prog" / "0000000001.pfm").exists()
     img = _read_hdr(out)
     assert img.shape == (24, 32, 3) and np.isfinite(img).all() and img.mean() > 0
+
+
+@pytest.mark.gpu
+def test_cli_time_budget_renders_whole_chunks(tmp_path):
+    """-t renders chunks of GrainSize * 1000 paths until the time is up
+    (src/nanogi.cpp:243-418): the sample count is a whole number of chunks
+    and the elapsed time reaches the budget."""
+    out = tmp_path / "img.pfm"
+    r = _cli("pt", CORNELL, str(out), "32", "24", "-t", "0.3", "--grain-size", "10")
+    assert r.returncode == 0, r.stderr
+    n = int(r.stdout.split("# of samples: ")[1].split()[0])
+    t = float(r.stdout.split("Elapsed time: ")[1].split()[0])
+    assert n >= 10000 and n % 10000 == 0 and t >= 0.3
+    img = _read_pfm(out)
+    assert img.shape == (24, 32, 3) and np.isfinite(img).all() and img.mean() > 0
