@@ -55,7 +55,7 @@ def build(verbose=False, jobs=None, lib_path=LIB, out_dir=OUT_DIR, extra_hip_fla
     lib_objs = objs[:len(lib_srcs)]
     cli_objs = objs[len(lib_srcs):]
     tmp = lib_path + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + lib_objs + ["-lpthread"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + lib_objs + ["-lz", "-lpthread"])
     os.replace(tmp, lib_path)
     if cli_objs and lib_path == LIB:
         _run([CXX, "-o", CLI] + cli_objs + ["-L", HERE, "-lngi", "-Wl,-rpath,$ORIGIN", "-lpthread"])

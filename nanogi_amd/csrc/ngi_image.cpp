@@ -5,12 +5,15 @@
 // reference's FreeImage (bottom-up scanlines) and tinyexr (explicit y flip,
 // basic.hpp:590-597) paths do:
 //   .hdr  Radiance RGBE, "-Y H +X W", run-length encoded scanlines
-//   .exr  OpenEXR scanline image, FLOAT channels B, G, R, no compression
+//   .exr  OpenEXR scanline image, FLOAT channels B, G, R, ZIP compression
+//         (blocks of 16 scanlines, as tinyexr writes them)
 //   .png  8-bit RGB, v -> clamp(int(v^(1/2.2) * 255), 0, 255) (basic.hpp:627-641),
-//         deflate "stored" blocks
+//         zlib-deflated scanlines
 //   .pfm  portable float map (bottom-up rows natively)
 #include <sys/stat.h>
+#include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +56,15 @@ uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0xFFFFFFFFu) {
     }
     for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
     return c;
+}
+
+// zlib stream (RFC 1950) of `in`.
+bool deflate_zlib(const std::vector<uint8_t>& in, std::vector<uint8_t>& out) {
+    uLongf n = compressBound((uLong)in.size());
+    out.resize(n);
+    if (compress2(out.data(), &n, in.data(), (uLong)in.size(), 6) != Z_OK) return false;
+    out.resize(n);
+    return true;
 }
 
 int write_file(const std::string& path, const std::vector<uint8_t>& bytes) {
@@ -149,7 +161,7 @@ int save_exr(const std::string& path, const double* film, int w, int h) {
     }
     ch.push_back(0);
     put_attr(o, "channels", "chlist", ch);
-    put_attr(o, "compression", "compression", {0});
+    put_attr(o, "compression", "compression", {3});   // ZIP: 16 scanlines per block
     std::vector<uint8_t> box;
     put_u32le(box, 0), put_u32le(box, 0), put_u32le(box, (uint32_t)(w - 1)), put_u32le(box, (uint32_t)(h - 1));
     put_attr(o, "dataWindow", "box2i", box);
@@ -165,23 +177,37 @@ int save_exr(const std::string& path, const double* film, int w, int h) {
     put_attr(o, "screenWindowCenter", "v2f", ctr);
     put_attr(o, "screenWindowWidth", "float", par);
     o.push_back(0);   // end of header
+    // Offset table, filled in as the blocks are written.
+    const int blocks = (h + 15) / 16;
     const size_t table = o.size();
-    const size_t line_bytes = 8 + 3 * 4 * (size_t)w;
-    for (int y = 0; y < h; ++y) {
-        const uint64_t off = table + 8 * (uint64_t)h + (uint64_t)y * line_bytes;
-        for (int k = 0; k < 8; ++k) o.push_back((uint8_t)(off >> (8 * k)));
-    }
-    for (int y = 0; y < h; ++y) {
-        put_u32le(o, (uint32_t)y);
-        put_u32le(o, (uint32_t)(3 * 4 * w));
-        const int fy = h - 1 - y;   // top row first
-        for (int c : {2, 1, 0}) {   // B, G, R
-            for (int x = 0; x < w; ++x) {
-                const float v = (float)film[3 * ((size_t)fy * w + x) + c];
-                memcpy(&bits, &v, 4);
-                put_u32le(o, bits);
+    o.resize(table + 8 * (size_t)blocks, 0);
+    std::vector<uint8_t> raw, tmp, z;
+    for (int b = 0; b < blocks; ++b) {
+        const int y0 = 16 * b, y1 = std::min(h, y0 + 16);
+        raw.clear();
+        for (int y = y0; y < y1; ++y) {
+            const int fy = h - 1 - y;   // top row first
+            for (int c : {2, 1, 0}) {   // B, G, R
+                for (int x = 0; x < w; ++x) {
+                    const float v = (float)film[3 * ((size_t)fy * w + x) + c];
+                    memcpy(&bits, &v, 4);
+                    put_u32le(raw, bits);
+                }
             }
         }
+        // OpenEXR's ZIP filter: even-indexed bytes then odd-indexed bytes,
+        // then each byte replaced by its difference to the previous one + 128.
+        const size_t n = raw.size(), half = (n + 1) / 2;
+        tmp.resize(n);
+        for (size_t i = 0; i < n; ++i) tmp[(i & 1) ? half + i / 2 : i / 2] = raw[i];
+        for (size_t i = n; i-- > 1;) tmp[i] = (uint8_t)(tmp[i] - tmp[i - 1] + 128);
+        if (!deflate_zlib(tmp, z)) return fail(NGI_ERR_IO, "Failed to save image : %s (zlib)", path.c_str());
+        const std::vector<uint8_t>& data = z.size() < n ? z : raw;   // readers take raw data when it is not smaller
+        const uint64_t off = o.size();
+        for (int k = 0; k < 8; ++k) o[table + 8 * (size_t)b + k] = (uint8_t)(off >> (8 * k));
+        put_u32le(o, (uint32_t)y0);
+        put_u32le(o, (uint32_t)data.size());
+        o.insert(o.end(), data.begin(), data.end());
     }
     return write_file(path, o);
 }
@@ -200,24 +226,8 @@ int save_png(const std::string& path, const double* film, int w, int h) {
                 raw.push_back((uint8_t)q);
             }
     }
-    // zlib stream of stored deflate blocks.
-    std::vector<uint8_t> z = {0x78, 0x01};
-    size_t pos = 0;
-    do {
-        const size_t n = std::min<size_t>(65535, raw.size() - pos);
-        const bool last = pos + n == raw.size();
-        z.push_back(last ? 1 : 0);
-        z.push_back((uint8_t)(n & 0xFF)), z.push_back((uint8_t)(n >> 8));
-        z.push_back((uint8_t)(~n & 0xFF)), z.push_back((uint8_t)((~n >> 8) & 0xFF));
-        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + n);
-        pos += n;
-    } while (pos < raw.size());
-    uint32_t a = 1, b = 0;
-    for (uint8_t c : raw) {
-        a = (a + c) % 65521u;
-        b = (b + a) % 65521u;
-    }
-    put_u32be(z, (b << 16) | a);
+    std::vector<uint8_t> z;
+    if (!deflate_zlib(raw, z)) return fail(NGI_ERR_IO, "Failed to save image : %s (zlib)", path.c_str());
     std::vector<uint8_t> o = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
     auto chunk = [&](const char* type, const std::vector<uint8_t>& data) {
         put_u32be(o, (uint32_t)data.size());

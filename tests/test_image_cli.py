@@ -80,16 +80,36 @@ def _read_exr(path):
         attrs[name] = data[type_end + 5:type_end + 5 + size]
         pos = type_end + 5 + size
     pos += 1
-    assert attrs["compression"] == b"\0"
+    comp = attrs["compression"][0]
+    assert comp in (0, 3)   # none, or ZIP over blocks of 16 scanlines
     x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"])
     w, h = x1 - x0 + 1, y1 - y0 + 1
-    offsets = struct.unpack(f"<{h}Q", data[pos:pos + 8 * h])
+    lines = 16 if comp == 3 else 1
+    nblocks = (h + lines - 1) // lines
+    offsets = struct.unpack(f"<{nblocks}Q", data[pos:pos + 8 * nblocks])
     img = np.zeros((h, w, 3), dtype=np.float32)
     for off in offsets:
         y, size = struct.unpack("<iI", data[off:off + 8])
-        planes = np.frombuffer(data[off + 8:off + 8 + size], dtype="<f4").reshape(3, w)
-        img[y] = planes[::-1].T   # B, G, R -> R, G, B
+        n = min(lines, h - y)
+        buf = data[off + 8:off + 8 + size]
+        if comp == 3 and size < n * 3 * 4 * w:
+            buf = _exr_unzip(buf)
+        planes = np.frombuffer(buf, dtype="<f4").reshape(n, 3, w)
+        img[y:y + n] = planes[:, ::-1].transpose(0, 2, 1)   # B, G, R -> R, G, B
     return img[::-1]
+
+
+def _exr_unzip(buf):
+    """OpenEXR ZIP block (file-format specification): inflate, undo the
+    byte-delta predictor (d = t[i] - t[i-1] + 128), then interleave the
+    first half (even-indexed bytes) with the second (odd-indexed bytes)."""
+    d = np.frombuffer(zlib.decompress(buf), dtype=np.uint8).astype(np.int64)
+    d[1:] -= 128
+    t = (np.cumsum(d) & 0xFF).astype(np.uint8)
+    out = np.empty_like(t)
+    half = (len(t) + 1) // 2
+    out[0::2], out[1::2] = t[:half], t[half:]
+    return out.tobytes()
 
 
 def _read_png(path):
@@ -125,6 +145,39 @@ def test_exr_round_trip(tmp_path):
     f = _film()
     ngi.save_image(tmp_path / "sub" / "a.exr", f)   # parent directory is created
     np.testing.assert_array_equal(_read_exr(tmp_path / "sub" / "a.exr"), f.astype(np.float32))
+
+
+@pytest.mark.parametrize("h", [16, 37, 1])
+def test_exr_zip_blocks_of_a_smooth_image(tmp_path, h):
+    """A smooth film compresses: ZIP blocks of 16 scanlines (the last one
+    partial) decode to the exact FP32 values, and the file is smaller than
+    the uncompressed pixels."""
+    w = 64
+    yy, xx = np.mgrid[0:h, 0:w]
+    f = np.stack([0.01 * xx + 0.02 * yy, np.full((h, w), 0.5), 1.0 / (1.0 + xx + yy)], axis=2)
+    path = tmp_path / "s.exr"
+    ngi.save_image(path, f)
+    np.testing.assert_array_equal(_read_exr(path), f.astype(np.float32))
+    if h > 1:
+        assert path.stat().st_size < 12 * w * h
+
+
+def test_png_is_deflated(tmp_path):
+    f = np.full((40, 50, 3), 0.25)
+    ngi.save_image(tmp_path / "c.png", f)
+    assert (tmp_path / "c.png").stat().st_size < 40 * 50 * 3 // 4
+    assert np.all(_read_png(tmp_path / "c.png") == int(0.25 ** (1 / 2.2) * 255))
+
+
+def test_png_reads_back_with_pillow(tmp_path):
+    """An independent decoder sees the same upright, gamma-encoded pixels."""
+    Image = pytest.importorskip("PIL.Image")
+    h, w = 37, 50
+    yy, xx = np.mgrid[0:h, 0:w]
+    f = np.stack([xx / w, yy / h, np.full((h, w), 0.3)], axis=2)
+    ngi.save_image(tmp_path / "g.png", f)
+    ref = np.clip((np.power(f.astype(np.float32).astype(np.float64), 1 / 2.2) * 255).astype(int), 0, 255)[::-1]
+    np.testing.assert_array_equal(np.asarray(Image.open(tmp_path / "g.png")), ref)
 
 
 def test_hdr_round_trip(tmp_path):
