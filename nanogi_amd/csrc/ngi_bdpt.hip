@@ -544,31 +544,50 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
         // Every lane reads the other lanes' vertex records below.
         __threadfence_block();
         // ---- strategies ----
-        // The strategies (s, t) of all the wavefront's samples form one pool,
+        // The strategies (s, t) of all the wavefront's samples are pooled and
         // evaluated 64 at a time: lane j of a round takes pool entry base + j,
         // which belongs to lane `owner` (whose records it reads) as its k-th
         // strategy in the reference's order (n ascending, then s).  A round
         // keeps every lane on a connection regardless of how many strategies
-        // each sample has.
+        // each sample has.  Pooling per path length n gives rounds whose MIS
+        // walks all take n steps (Cornell box 131 -> 158 M samples/s); with
+        // Russian roulette the per-length pools are small, so a wave pools by
+        // length only when that costs at most a quarter more rounds.
         const int nmax = (a.max_nv >= 0 && a.max_nv < nsub[0] + nsub[1]) ? a.max_nv : nsub[0] + nsub[1];
-        const uint32_t K = have ? strategy_count(nsub[0], nsub[1], nmax) : 0u;
-        uint32_t incl = K;   // inclusive prefix sum over the lanes
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off);
-            if ((int)lane >= off) incl += y;
-        }
-        const uint32_t total = __shfl(incl, 63);
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t g = base + lane < total ? base + lane : total - 1;
-            int owner = 0;   // the first lane whose prefix sum exceeds g
-            for (int step = 32; step > 0; step >>= 1)
-                if (__shfl(incl, owner + step - 1) <= g) owner += step;
-            const uint32_t k = g - (__shfl(incl, owner) - __shfl(K, owner));
-            const int oL = __shfl(nsub[0], owner), oE = __shfl(nsub[1], owner), om = __shfl(nmax, owner);
-            if (base + lane < total) {
-                int n = 0, s = 0;
-                strategy_at(k, oL, oE, om, n, s);
-                eval_strategy<FilmT, WIDTH, COUNT>(a, c, mem, gl - lane + (uint32_t)owner, s, n, my_stk, cnt);
+        int wmax = have ? nmax : 0;   // longest path length of the wave
+        for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
+        auto wave_sum = [](uint32_t v) {
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            return v;
+        };
+        const uint32_t Kall = have ? strategy_count(nsub[0], nsub[1], nmax) : 0u;
+        const uint32_t rounds_all = (wave_sum(Kall) + 63u) / 64u;
+        uint32_t rounds_n = 0;
+        for (int n = 2; n <= wmax; ++n)
+            rounds_n += (wave_sum((have && n <= nmax) ? (uint32_t)strategies_of_length(n, nsub[0], nsub[1]) : 0u) + 63u) / 64u;
+        const bool by_length = 4u * rounds_n <= 5u * rounds_all;
+        for (int n = by_length ? 2 : 0; n <= (by_length ? wmax : 0); ++n) {   // n = 0: the single pool
+            const uint32_t K = !have ? 0u
+                               : (n == 0 ? Kall : (n <= nmax ? (uint32_t)strategies_of_length(n, nsub[0], nsub[1]) : 0u));
+            uint32_t incl = K;   // inclusive prefix sum over the lanes
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(incl, off);
+                if ((int)lane >= off) incl += y;
+            }
+            const uint32_t total = __shfl(incl, 63);
+            for (uint32_t base = 0; base < total; base += 64) {
+                const uint32_t g = base + lane < total ? base + lane : total - 1;
+                int owner = 0;   // the first lane whose prefix sum exceeds g
+                for (int step = 32; step > 0; step >>= 1)
+                    if (__shfl(incl, owner + step - 1) <= g) owner += step;
+                const uint32_t k = g - (__shfl(incl, owner) - __shfl(K, owner));
+                const int oL = __shfl(nsub[0], owner), oE = __shfl(nsub[1], owner), om = __shfl(nmax, owner);
+                if (base + lane < total) {
+                    int sn = n, ss = 0;
+                    if (n == 0) strategy_at(k, oL, oE, om, sn, ss);
+                    else ss = (n - oE > 0 ? n - oE : 0) + (int)k;
+                    eval_strategy<FilmT, WIDTH, COUNT>(a, c, mem, gl - lane + (uint32_t)owner, ss, sn, my_stk, cnt);
+                }
             }
         }
     }
