@@ -40,6 +40,11 @@ namespace ngi {
 #ifndef NGI_BD_WAVES
 #define NGI_BD_WAVES 4
 #endif
+// Strategies are pooled per path length when that takes at most this many
+// quarters of the rounds of a single pool (5: a quarter more).
+#ifndef NGI_BD_POOL_QUARTERS
+#define NGI_BD_POOL_QUARTERS 5u
+#endif
 // Vertices per subpath; the reference has no cap with -m -1 (with Russian
 // roulette at 0.5 a longer subpath has probability 2^-62).
 #define NGI_BDPT_MAX_VERTICES 64
@@ -565,7 +570,7 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
         uint32_t rounds_n = 0;
         for (int n = 2; n <= wmax; ++n)
             rounds_n += (wave_sum((have && n <= nmax) ? (uint32_t)strategies_of_length(n, nsub[0], nsub[1]) : 0u) + 63u) / 64u;
-        const bool by_length = 4u * rounds_n <= 5u * rounds_all;
+        const bool by_length = 4u * rounds_n <= NGI_BD_POOL_QUARTERS * rounds_all;
         for (int n = by_length ? 2 : 0; n <= (by_length ? wmax : 0); ++n) {   // n = 0: the single pool
             const uint32_t K = !have ? 0u
                                : (n == 0 ? Kall : (n <= nmax ? (uint32_t)strategies_of_length(n, nsub[0], nsub[1]) : 0u));
