@@ -74,14 +74,15 @@ def philox(ctr, key):
     return out
 
 
-def bsdf(prim, type_bits, wi, u=(0.0, 0.0, 0.0), wo=None, force_degenerated=True):
+def bsdf(prim, type_bits, wi, u=(0.0, 0.0, 0.0), wo=None, force_degenerated=True, le=False):
     """(wo, pdf, fs) of the oracle's SampleDirection/EvaluateDirection[PDF]
-    at a surface with normal +z; wo given -> evaluate only."""
+    at a surface with normal +z; wo given -> evaluate only; le selects the
+    light-to-eye transport direction of EvaluateDirection."""
     import nanogi_amd as ngi
     L = lib()
     if not hasattr(L.oracle_bsdf, "_set"):
         L.oracle_bsdf.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.oracle_bsdf.restype = None
         L.oracle_bsdf._set = True
     p = np.ascontiguousarray(np.asarray(prim, dtype=ngi.PRIM_DTYPE).reshape(1))
@@ -90,7 +91,7 @@ def bsdf(prim, type_bits, wi, u=(0.0, 0.0, 0.0), wo=None, force_degenerated=True
     wo_a = np.asarray(wo if wo is not None else (0, 0, 0), dtype=np.float64)
     out = np.zeros(7, dtype=np.float64)
     L.oracle_bsdf(p.ctypes.data, int(type_bits), wi_a.ctypes.data, u_a.ctypes.data, wo_a.ctypes.data,
-                  int(wo is not None), int(force_degenerated), out.ctypes.data)
+                  int(wo is not None), int(force_degenerated), int(le), out.ctypes.data)
     return out[:3], out[3], out[4:]
 
 

@@ -1168,9 +1168,10 @@ extern "C" {
 // Test hook: SampleDirection / EvaluateDirectionPDF / EvaluateDirection of
 // one primitive at a surface whose geometric and shading normals are +z
 // (so local and world frames coincide).  out = {wo.x, wo.y, wo.z, pdf,
-// fs.r, fs.g, fs.b}; force_degenerated selects the sampling-time branch.
+// fs.r, fs.g, fs.b}; force_degenerated selects the sampling-time branch and
+// le the transport direction of EvaluateDirection (TransportDirection::LE).
 void oracle_bsdf(const ngi_primitive* prim, int type, const double wi[3], const double u[3], const double wo_in[3], int use_wo_in,
-                 int force_degenerated, double out[7]) {
+                 int force_degenerated, int le, double out[7]) {
     Prim P;
     P.p = *prim;
     Geom g;
@@ -1183,7 +1184,7 @@ void oracle_bsdf(const ngi_primitive* prim, int type, const double wi[3], const 
     if (use_wo_in) wo = V(wo_in[0], wo_in[1], wo_in[2]);
     else SampleDirection(P, type, g, w, u[0], u[1], u[2], wo);
     const double pdf = EvaluateDirectionPDF(P, type, g, w, wo);
-    const V fs = EvaluateDirection(P, type, g, w, wo, force_degenerated != 0);
+    const V fs = EvaluateDirection(P, type, g, w, wo, force_degenerated != 0, le != 0);
     out[0] = wo.x, out[1] = wo.y, out[2] = wo.z, out[3] = pdf, out[4] = fs.x, out[5] = fs.y, out[6] = fs.z;
 }
 

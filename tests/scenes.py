@@ -169,3 +169,71 @@ def pixel_plane_points(W, h, fov, sub=1):
     c = (np.arange(W * sub) + 0.5) / (W * sub) * 2 - 1
     X, Z = np.meshgrid(c * t * 0.5 * h, c * t * 0.5 * h)
     return X.ravel(), Z.ravel()
+
+
+def _cube(B, pid, lo, hi, inward):
+    """Six faces of the axis-aligned box [lo, hi] with outward (or inward) winding normals."""
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    c, h = (lo + hi) * 0.5, (hi - lo) * 0.5
+    sgn = -1.0 if inward else 1.0
+    for ax in range(3):
+        a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+        for side in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[ax] = side * sgn
+            ctr = c.copy()
+            ctr[ax] += side * h[ax]
+            axis = np.zeros(3)
+            axis[a1] = 1.0
+            # rect() spans `axis` by ha and cross(n, axis) by hb
+            hb = h[a2]
+            B.rect(pid, ctr, n, axis, h[a1], hb)
+
+
+def glass_cube_in_light_box(spec_type, eta=1.5, Le=(2.0, 1.0, 0.5), look=(0.0, 0.0, -1.0), fov=30.0):
+    """Camera at the centre of a unit glass cube (S primitive, Eta1 = 1
+    outside / Eta2 = eta inside, outward normals) inside a closed box of
+    3 x 3 x 3 inward-facing [L]-only walls.  With -m 3 a camera ray that
+    leaves the glass reaches a wall: pt sees eta^2 Le times the probability
+    of transmission (radiance scales as n^2 across an interface)."""
+    B = SceneBuilder()
+    g = B.prim(type=ngi.PRIM_S, spec_type=spec_type, S_R=(1, 1, 1), S_eta1=1.0, S_eta2=eta)
+    _cube(B, g, (0, 0, 0), (1, 1, 1), inward=False)
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
+    _cube(B, li, (-1, -1, -1), (2, 2, 2), inward=True)
+    look = np.asarray(look, dtype=np.float64)
+    B.camera((0.5, 0.5, 0.5), tuple(np.array([0.5, 0.5, 0.5]) + look), fov=fov)
+    return B.scene()
+
+
+def slab_furnace(eta=1.5, Le=(1.0, 0.7, 0.4)):
+    """Closed [L]-only box (2 x 2 x 2, inward normals) with a Fresnel glass slab
+    (1.2 x 0.3 x 1.2) in the middle and the camera above it looking down.
+    Entering and leaving the slab scale radiance by (1/eta)^2 and eta^2, and a
+    reflection's Fr / pdf is 1, so every pt path that reaches a wall carries
+    exactly Le (paths cut off by -m carry 0)."""
+    B = SceneBuilder()
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
+    _cube(B, li, (-1, -1, -1), (1, 1, 1), inward=True)
+    g = B.prim(type=ngi.PRIM_S, spec_type=ngi.SPEC_FRESNEL, S_R=(1, 1, 1), S_eta1=1.0, S_eta2=eta)
+    _cube(B, g, (-0.6, -0.15, -0.6), (0.6, 0.15, 0.6), inward=False)
+    B.camera((0.1, 0.8, 0.2), (0.0, 0.0, 0.0), up=(0, 0, -1), fov=60.0)
+    return B.scene()
+
+
+def glass_room(spec_type=None, eta=1.5, Le=(6.0, 5.0, 4.0), rho=(0.7, 0.5, 0.3)):
+    """A unit glass box (outward normals) holding the camera and a diffuse
+    floor; one 0.4 x 0.4 area light outside, above the box.  Light reaches
+    the floor only through the glass top, so path tracing (EL: eta^2 on
+    leaving the glass) and light tracing (LE: no scaling) meet only if both
+    transport conventions are right."""
+    B = SceneBuilder()
+    g = B.prim(type=ngi.PRIM_S, spec_type=spec_type if spec_type is not None else ngi.SPEC_REFRACTION,
+               S_R=(1, 1, 1), S_eta1=1.0, S_eta2=eta)
+    _cube(B, g, (0, 0, 0), (1, 1, 1), inward=False)
+    fl = B.prim(type=ngi.PRIM_D, D_R=rho)
+    B.rect(fl, (0.5, 0.05, 0.5), (0, 1, 0), (1, 0, 0), 0.49, 0.49)
+    li = B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_AREA, Le=Le)
+    B.rect(li, (0.5, 1.6, 0.5), (0, -1, 0), (1, 0, 0), 0.2, 0.2)
+    B.camera((0.5, 0.9, 0.9), (0.5, 0.05, 0.4), fov=50.0)
+    return B.scene()
