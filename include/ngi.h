@@ -136,7 +136,10 @@ int ngi_scene_get_desc(const ngi_scene* scene, ngi_scene_desc* out);
 
 /* Replace per-primitive reflectance (D.R, or G.R for glossy) and emitter
  * radiance (L.*.Le): arrays of num_primitives*3 doubles, either may be NULL.
- * Entries of primitives without the matching type bit are ignored. */
+ * Entries of primitives without the matching type bit are ignored.  The
+ * device tables are rewritten at the next render, after every enqueued
+ * render or batch frame that reads them has finished, so a frame enqueued
+ * before this call renders with the old values. */
 int ngi_scene_set_params(ngi_scene* scene, const double* reflectance, const double* Le);
 
 /* ------------------------------------------------------------------ */
@@ -163,7 +166,11 @@ typedef struct ngi_bvh_info {
     double build_ms;
 } ngi_bvh_info;
 
-/* Build the host BVH (deterministic for a given scene and options). */
+/* Build the host BVH (deterministic for a given scene and options).
+ * Rebuilding releases the scene's device copy after waiting for every
+ * kernel that reads it (its own stream and the streams of batches holding
+ * the scene).  Sessions of the scene stay valid: their films keep their
+ * contents and the scene is uploaded again at the session's next call. */
 int ngi_scene_build(ngi_scene* scene, const ngi_bvh_opts* opts);
 int ngi_scene_bvh_info(const ngi_scene* scene, ngi_bvh_info* out);
 /* Copy out the flattened node records (info.num_nodes * info.node_bytes

@@ -16,7 +16,20 @@
 // sum when vertex j's reflectance first multiplies the path,
 //     dL/dR_m += sum_{j : prim_j = m} (S - P_j) / R_m      (per channel)
 //     dL/dLe_l += sum_{k : emitter_k = l} g_k C_k / Le_l.
-// Channels with R_m = 0 or Le_l = 0 get no gradient.
+// A parameter channel that is zero takes no part in a division: the replay
+// runs with every zero reflectance or radiance channel replaced by 1 (path
+// sampling does not depend on them, so the vertices are unchanged), and each
+// path counts per channel the zero-reflectance vertices it has passed, z.
+// The true value of a replayed contribution C~_k is C~_k if z_k = 0 and its
+// emitter's Le is non-zero, else 0.  By the product rule:
+//   - a record j with non-zero R gets (S_A - P_A,j) / R, S_A summing the
+//     contributions with z = 0 and non-zero Le;
+//   - the first zero-R record of a channel gets S_B, the sum of the
+//     contributions with z = 1 and non-zero Le (its own factor is the one
+//     that vanished; the replayed factor 1 leaves the rest);
+//   - later zero-R records get 0 (another vanished factor remains);
+//   - an emitter channel gets g C~ / Le~ from every contribution with z = 0
+//     (Le~ = 1 where Le = 0).
 //
 // Layout: the persistent engine's regenerating lanes; per-lane vertex
 // records (prim, P_j) in HBM as [record][lane] float4; per-block FP64 LDS
@@ -41,9 +54,18 @@ namespace ngi {
 #endif
 #define NGI_ADJ_LDS_PRIMS 256
 
+// Zero-parameter mask per primitive: bits 0-2 reflectance channel is 0,
+// bits 3-5 radiance channel is 0 (the replay tables hold 1 there).
+#define NGI_ADJ_RZERO 1u
+#define NGI_ADJ_LEZERO 8u
+// Record word w: primitive in the low 29 bits, bits 29-31 = this record is
+// the first zero-reflectance vertex of channel 0/1/2 on its path.
+#define NGI_ADJ_PRIM_MASK 0x1fffffffu
+
 struct AdjArgs {
     const float* dLdI;     // W*H*3
     float scale;           // W*H / N
+    const uint8_t* zmask;  // per primitive, NGI_ADJ_RZERO / NGI_ADJ_LEZERO bits
     float4* rec;           // [max_rec][lanes]
     uint32_t lanes;
     int max_rec;
@@ -77,31 +99,50 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
     const uint32_t gid = blockIdx.x * NGI_ADJ_BLOCK + threadIdx.x;
     uint32_t* my_stk = stk + threadIdx.x;
 
-    // Path state (as ProcessSample_PT/_PTDirect) plus the adjoint sums.
-    f3 T(0.0f), wi(0.0f), S(0.0f);
+    // Path state (as ProcessSample_PT/_PTDirect) plus the adjoint sums:
+    // SA / SB = weighted contributions with no / one zero-reflectance vertex
+    // before them, zc = per-channel count of such vertices (2 bits each,
+    // saturating at 2).
+    f3 T(0.0f), wi(0.0f), SA(0.0f), SB(0.0f);
     Surf<float> s;
     int type = 0, prim = 0, nv = 0, pixel = 0, nrec = 0;
-    uint32_t c0 = 0, c1 = 0;
+    uint32_t c0 = 0, c1 = 0, zc = 0;
     bool active = false, exhausted = false;
 
     auto contribute = [&](int pix, f3 C, int emitter_prim) {
         const float* gp = g.dLdI + 3 * (size_t)pix;
-        const f3 gC(gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale);
-        S = S + gC;
+        const float gC[3] = {gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale};
         const DevPrim<float>& E = prims[emitter_prim];
-        if (E.Le[0] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 0, gC.x / E.Le[0]);
-        if (E.Le[1] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 1, gC.y / E.Le[1]);
-        if (E.Le[2] != 0.0f) grad_add(sL, g.gLe, nlds, emitter_prim, 2, gC.z / E.Le[2]);
+        const uint32_t zm = g.zmask[emitter_prim];
+        float sa[3] = {SA.x, SA.y, SA.z}, sb[3] = {SB.x, SB.y, SB.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t z = (zc >> (2 * c)) & 3u;
+            const bool le_zero = (zm & (NGI_ADJ_LEZERO << c)) != 0;
+            if (z == 0) {
+                grad_add(sL, g.gLe, nlds, emitter_prim, c, gC[c] / E.Le[c]);   // Le~ = 1 where Le = 0
+                if (!le_zero) sa[c] += gC[c];
+            } else if (z == 1 && !le_zero) {
+                sb[c] += gC[c];
+            }
+        }
+        SA = f3(sa[0], sa[1], sa[2]);
+        SB = f3(sb[0], sb[1], sb[2]);
     };
     auto finish_path = [&]() {
+        const float sa[3] = {SA.x, SA.y, SA.z}, sb[3] = {SB.x, SB.y, SB.z};
         for (int r = 0; r < nrec; ++r) {
             const float4 e = g.rec[(size_t)r * g.lanes + gid];
-            const int m = __float_as_int(e.w);
+            const uint32_t w = __float_as_uint(e.w);
+            const int m = (int)(w & NGI_ADJ_PRIM_MASK);
             const DevPrim<float>& P = prims[m];
-            const f3 d = S - f3(e.x, e.y, e.z);
-            if (P.R[0] != 0.0f) grad_add(sR, g.gR, nlds, m, 0, d.x / P.R[0]);
-            if (P.R[1] != 0.0f) grad_add(sR, g.gR, nlds, m, 1, d.y / P.R[1]);
-            if (P.R[2] != 0.0f) grad_add(sR, g.gR, nlds, m, 2, d.z / P.R[2]);
+            const uint32_t zm = g.zmask[m];
+            const float pa[3] = {e.x, e.y, e.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (!(zm & (NGI_ADJ_RZERO << c))) grad_add(sR, g.gR, nlds, m, c, (sa[c] - pa[c]) / P.R[c]);
+                else if (w & (1u << (29 + c))) grad_add(sR, g.gR, nlds, m, c, sb[c]);
+            }
         }
     };
 
@@ -125,7 +166,9 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
                         s.degenerate = true;
                         wi = f3(0.0f);
                         nv = 1;
-                        S = f3(0.0f);
+                        SA = f3(0.0f);
+                        SB = f3(0.0f);
+                        zc = 0;
                         nrec = 0;
                         active = true;
                     }
@@ -138,7 +181,16 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
         bool alive = nv < max_nv;
         if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && nrec < g.max_rec) {
             // Vertex reflectance enters here (NEE fsE, then the BSDF weight).
-            g.rec[(size_t)nrec * g.lanes + gid] = make_float4(S.x, S.y, S.z, __int_as_float(prim));
+            const uint32_t zm = g.zmask[prim];
+            uint32_t w = (uint32_t)prim;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (!(zm & (NGI_ADJ_RZERO << c))) continue;
+                const uint32_t z = (zc >> (2 * c)) & 3u;
+                if (z == 0) w |= 1u << (29 + c);
+                if (z < 2) zc += 1u << (2 * c);
+            }
+            g.rec[(size_t)nrec * g.lanes + gid] = make_float4(SA.x, SA.y, SA.z, __uint_as_float(w));
             nrec++;
         }
         if (alive && DIRECT && sc.num_lights > 0) {
@@ -267,7 +319,35 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     g.scale = total ? (float)((double)npix / (double)total) : 0.0f;
     std::vector<float> dl(3 * (size_t)npix);
     for (size_t i = 0; i < dl.size(); ++i) dl[i] = (float)dLdI[i];
+    if ((uint32_t)nprims > NGI_ADJ_PRIM_MASK) {
+        cleanup();
+        return fail(NGI_ERR_INVALID_ARG, "adjoint: at most 2^29 - 1 primitives");
+    }
+    // Replay tables: the scene's FP32 primitive and light records with every
+    // zero reflectance (D/G) or radiance (L) channel set to 1, and the mask of
+    // those channels.
+    const DevScene& view = scene->dev->view;
+    std::vector<DevPrim<float>> rp(std::max(nprims, 1));
+    std::vector<DevLight<float>> rl(std::max(view.num_lights, 1));
+    std::vector<uint8_t> zm(std::max(nprims, 1), 0);
+    if (hipMemcpy(rp.data(), view.prims_f, (size_t)nprims * sizeof(DevPrim<float>), hipMemcpyDeviceToHost) != hipSuccess ||
+        (view.num_lights &&
+         hipMemcpy(rl.data(), view.lights_f, (size_t)view.num_lights * sizeof(DevLight<float>), hipMemcpyDeviceToHost) != hipSuccess)) {
+        cleanup();
+        return fail(NGI_ERR_DEVICE, "adjoint: cannot read the parameter tables");
+    }
+    for (int i = 0; i < nprims; ++i) {
+        const int ty = scene->prims[i].type;
+        for (int c = 0; c < 3; ++c) {
+            if ((ty & (NGI_PRIM_D | NGI_PRIM_G)) && rp[i].R[c] == 0.0f) zm[i] |= (uint8_t)(NGI_ADJ_RZERO << c), rp[i].R[c] = 1.0f;
+            if ((ty & NGI_PRIM_L) && rp[i].Le[c] == 0.0f) zm[i] |= (uint8_t)(NGI_ADJ_LEZERO << c), rp[i].Le[c] = 1.0f;
+        }
+    }
+    for (int l = 0; l < view.num_lights; ++l)
+        for (int c = 0; c < 3; ++c)
+            if (zm[rl[l].prim] & (NGI_ADJ_LEZERO << c)) rl[l].Le[c] = 1.0f;
     void *d_dl = nullptr, *d_rec = nullptr, *d_gR = nullptr, *d_gL = nullptr, *d_next = nullptr;
+    void *d_rp = nullptr, *d_rl = nullptr, *d_zm = nullptr;
 #define NGI_ADJ_ALLOC(ptr, bytes)                                   \
     do {                                                            \
         hipError_t e_ = hipMalloc(&ptr, bytes);                     \
@@ -282,8 +362,14 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     NGI_ADJ_ALLOC(d_gR, (size_t)nprims * 3 * 8 + 8);
     NGI_ADJ_ALLOC(d_gL, (size_t)nprims * 3 * 8 + 8);
     NGI_ADJ_ALLOC(d_next, 8);
+    NGI_ADJ_ALLOC(d_rp, rp.size() * sizeof(DevPrim<float>));
+    NGI_ADJ_ALLOC(d_rl, rl.size() * sizeof(DevLight<float>));
+    NGI_ADJ_ALLOC(d_zm, zm.size());
 #undef NGI_ADJ_ALLOC
-    if (hipMemcpyAsync(d_dl, dl.data(), dl.size() * 4, hipMemcpyHostToDevice, str) != hipSuccess) {
+    if (hipMemcpyAsync(d_dl, dl.data(), dl.size() * 4, hipMemcpyHostToDevice, str) != hipSuccess ||
+        hipMemcpyAsync(d_rp, rp.data(), rp.size() * sizeof(DevPrim<float>), hipMemcpyHostToDevice, str) != hipSuccess ||
+        hipMemcpyAsync(d_rl, rl.data(), rl.size() * sizeof(DevLight<float>), hipMemcpyHostToDevice, str) != hipSuccess ||
+        hipMemcpyAsync(d_zm, zm.data(), zm.size(), hipMemcpyHostToDevice, str) != hipSuccess) {
         cleanup();
         return fail(NGI_ERR_DEVICE, "adjoint: upload failed");
     }
@@ -294,7 +380,10 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     g.rec = reinterpret_cast<float4*>(d_rec);
     g.gR = reinterpret_cast<double*>(d_gR);
     g.gLe = reinterpret_cast<double*>(d_gL);
+    g.zmask = reinterpret_cast<const uint8_t*>(d_zm);
     RenderArgs a = S.args(nullptr);
+    a.sc.prims_f = reinterpret_cast<const DevPrim<float>*>(d_rp);
+    a.sc.lights_f = reinterpret_cast<const DevLight<float>*>(d_rl);
     a.path_begin = 0;
     a.path_count = total;
     a.sample_base = 0;

@@ -404,6 +404,11 @@ int Batch::create(ngi_scene* const* sc, const ngi_render_params* p, int count, c
     NGI_HIP_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     NGI_HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
     NGI_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // set_params / build on a scene of the batch wait for this stream before
+    // they overwrite or free the scene's device arrays.
+    for (ngi_scene* sc : scenes)
+        if (std::find(sc->user_streams.begin(), sc->user_streams.end(), (void*)stream) == sc->user_streams.end())
+            sc->user_streams.push_back((void*)stream);
     NGI_HIP_CHECK(hipEventCreate(&ev0));
     NGI_HIP_CHECK(hipEventCreate(&ev1));
     if ((st = dev_alloc(&film, film_off.back())) || (st = dev_alloc(&d_args, count)) || (st = dev_alloc(&d_off, count + 1)) ||
@@ -582,6 +587,10 @@ int Batch::get_times(double* scene_ms, ngi_stats* stats) {
 
 void Batch::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (ngi_scene* sc : scenes) {
+        auto& u = sc->user_streams;
+        u.erase(std::remove(u.begin(), u.end(), (void*)stream), u.end());
+    }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* q : {(void*)film, (void*)d_args, (void*)d_off, (void*)d_tune_off, (void*)d_chunk, (void*)d_tune_chunk, (void*)d_done,

@@ -49,6 +49,14 @@ static void fill_prim(const ngi_primitive& p, int light, DevPrim<T>& o) {
     o.s_eta2 = (T)p.S_eta2;
 }
 
+// Wait for every kernel that may read the scene's device arrays: the
+// scene's stream and the streams of the batches holding it.
+static int wait_users(HostScene& s) {
+    if (s.dev && s.dev->stream) NGI_HIP_CHECK(hipStreamSynchronize(s.dev->stream));
+    for (void* u : s.user_streams) NGI_HIP_CHECK(hipStreamSynchronize(reinterpret_cast<hipStream_t>(u)));
+    return NGI_OK;
+}
+
 // (Re)build the parameter-dependent tables: primitives and lights.
 static int upload_params(HostScene& s, DeviceScene& d) {
     const size_t np = s.prims.size();
@@ -98,7 +106,8 @@ static int upload_params(HostScene& s, DeviceScene& d) {
         NGI_HIP_CHECK(hipMalloc(&d.light_d_mem, std::max<size_t>(ld.size(), 1) * sizeof(DevLight<double>)));
         d.allocs.insert(d.allocs.end(), {d.prim_f_mem, d.prim_d_mem, d.light_f_mem, d.light_d_mem});
     }
-    NGI_HIP_CHECK(hipStreamSynchronize(d.stream));
+    int st = wait_users(s);   // a running frame must not see a half-updated table
+    if (st) return st;
     if (np) {
         NGI_HIP_CHECK(hipMemcpy(d.prim_f_mem, pf.data(), np * sizeof(pf[0]), hipMemcpyHostToDevice));
         NGI_HIP_CHECK(hipMemcpy(d.prim_d_mem, pd.data(), np * sizeof(pd[0]), hipMemcpyHostToDevice));
@@ -203,6 +212,7 @@ int device_prepare(HostScene& s) {
 
 void device_release(HostScene& s) {
     if (!s.dev) return;
+    (void)wait_users(s);
     DeviceScene* d = s.dev;
     if (d->stream) {
         (void)hipStreamSynchronize(d->stream);

@@ -55,6 +55,10 @@ struct HostScene {
     uint64_t params_version = 0;     // bumped by ngi_scene_set_params
 
     DeviceScene* dev = nullptr;
+    // Streams other than the scene's own whose kernels read its device
+    // arrays (one per live batch holding the scene).  Re-uploads and
+    // releases wait for them first.
+    std::vector<void*> user_streams;
 };
 
 // Error reporting (thread-local message returned by ngi_last_error).

@@ -111,7 +111,8 @@ struct Session {
     WavefrontState* wf = nullptr;
     uint32_t wf_cap_max = 1u << 25;   // default wavefront queues: largest capacity not refused by the allocator
 
-    hipStream_t stream() const { return scene->dev->stream; }
+    hipStream_t stream() const { return scene->dev->stream; }   // valid after ensure_device()
+    int ensure_device();
     int init(ngi_scene* sc, const ngi_render_params& pp);
     int reset();
     void destroy();

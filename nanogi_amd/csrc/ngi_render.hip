@@ -187,8 +187,18 @@ __global__ void zero_kernel(FilmT* p, size_t n) {
     if (i < n) p[i] = FilmT(0);
 }
 
+// A rebuilt scene (ngi_scene_build) has released its device copy and its
+// stream; the session's own buffers survive, and the scene is re-uploaded
+// before the session enqueues anything again.
+int Session::ensure_device() {
+    if (scene->dev) return NGI_OK;
+    return device_prepare(*scene);
+}
+
 int Session::zero(void* p, size_t n_elems, bool f64) {
     if (n_elems == 0) return NGI_OK;
+    int st = ensure_device();
+    if (st) return st;
     const unsigned grid = (unsigned)((n_elems + 255) / 256);
     if (f64) zero_kernel<double><<<grid, 256, 0, stream()>>>(reinterpret_cast<double*>(p), n_elems);
     else zero_kernel<float><<<grid, 256, 0, stream()>>>(reinterpret_cast<float*>(p), n_elems);
@@ -271,6 +281,8 @@ int Session::timed_end(TimedLaunch& t) {
 }
 
 int Session::sync() {
+    int st = ensure_device();
+    if (st) return st;
     NGI_HIP_CHECK(hipStreamSynchronize(stream()));
     for (auto& t : pending) {
         float ms = 0;

@@ -635,15 +635,51 @@ struct Counts {
     long ext = 0, shadow = 0;
 };
 
-// Adjoint bookkeeping of one path (see oracle_render_adjoint).
+// Adjoint bookkeeping of one path (see oracle_render_adjoint).  The path
+// is replayed in a scene whose zero reflectance and radiance channels are
+// set to 1; Rt / Let hold the true values (3 per primitive).
 struct Adjoint {
     const double* dLdI = nullptr;   // W*H*3
     double scale = 1.0;             // W*H / N
     std::vector<double>* gR = nullptr;
     std::vector<double>* gLe = nullptr;
-    V S;
-    std::vector<std::pair<int, V>> rec;   // (primitive, S when its reflectance entered)
+    const std::vector<double>* Rt = nullptr;
+    const std::vector<double>* Let = nullptr;
+    std::vector<int> rec;   // primitives whose reflectance multiplied the path, in order
+    struct Contribution {
+        size_t nrec;        // records before it
+        int emitter;
+        V gc;               // dL/dI * scale * replayed contribution
+    };
+    std::vector<Contribution> contrib;
 };
+
+// Product rule over one replayed path.  A contribution C~ with replayed
+// factors R~_i (i < nrec) and Le~ is, in the true parameters,
+//   C = C~ * prod_i (R_i / R~_i) * (Le / Le~),
+// with R_i / R~_i = 1 for a non-zero channel and 0 for a zero one, so
+//   dC/dR_j = C~ / R~_j * prod_{i != j} (R_i / R~_i) * (Le / Le~)
+//   dC/dLe  = C~ / Le~  * prod_i (R_i / R~_i).
+static void adj_finish_path(Adjoint& A, const Scene& s) {
+    for (const Adjoint::Contribution& c : A.contrib) {
+        for (int k = 0; k < 3; ++k) {
+            int zeros = 0;
+            for (size_t i = 0; i < c.nrec; ++i) zeros += (*A.Rt)[3 * (size_t)A.rec[i] + k] == 0.0;
+            const double Le_t = (*A.Let)[3 * (size_t)c.emitter + k];
+            const double Le_r = s.prims[c.emitter].p.Le[k];
+            if (zeros == 0) (*A.gLe)[3 * (size_t)c.emitter + k] += c.gc[k] / Le_r;
+            if (Le_t == 0.0) continue;
+            for (size_t j = 0; j < c.nrec; ++j) {
+                const int m = A.rec[j];
+                const bool zj = (*A.Rt)[3 * (size_t)m + k] == 0.0;
+                if (zeros - (zj ? 1 : 0) != 0) continue;
+                const ngi_primitive& P = s.prims[m].p;
+                const double R_r = ((P.type & NGI_PRIM_D) ? P.D_R : P.G_R)[k];
+                (*A.gR)[3 * (size_t)m + k] += c.gc[k] / R_r;
+            }
+        }
+    }
+}
 
 // One path sample: ProcessSample_PT (direct = false) or _PTDirect (true).
 void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, std::vector<double>& film, Counts& cnt,
@@ -674,14 +710,11 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
         }
         const double* g = adj->dLdI + 3 * (size_t)pix;
         const V gc(g[0] * c.x * adj->scale, g[1] * c.y * adj->scale, g[2] * c.z * adj->scale);
-        adj->S = adj->S + gc;
-        const double* Le = s.prims[emitter].p.Le;
-        for (int k = 0; k < 3; ++k)
-            if (Le[k] != 0) (*adj->gLe)[3 * (size_t)emitter + k] += gc[k] / Le[k];
+        adj->contrib.push_back(Adjoint::Contribution{adj->rec.size(), emitter, gc});
     };
     if (adj) {
-        adj->S = V();
         adj->rec.clear();
+        adj->contrib.clear();
     }
     // Sensor (SampleEmitter(E): the last sensor, pdf 1) and its position.
     const Prim& E = s.prims[s.sensor];
@@ -697,7 +730,7 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
     const int maxv = (prm.max_num_vertices < 0 || prm.max_num_vertices > 1024) ? 1024 : prm.max_num_vertices;
     for (;;) {
         if (numVertices >= maxv) break;
-        if (adj && (type & (NGI_PRIM_D | NGI_PRIM_G))) adj->rec.emplace_back((int)(prim - &s.prims[0]), adj->S);
+        if (adj && (type & (NGI_PRIM_D | NGI_PRIM_G))) adj->rec.push_back((int)(prim - &s.prims[0]));
         if (direct && !s.lights.empty()) {
             const Q q = rng((uint32_t)numVertices, 1);
             const int n = (int)s.lights.size();
@@ -762,14 +795,7 @@ void process_sample(const Scene& s, const ngi_render_params& prm, uint64_t id, s
         wi = -wo;
         numVertices++;
     }
-    if (adj) {
-        for (const auto& r : adj->rec) {
-            const ngi_primitive& P = s.prims[r.first].p;
-            const double* R = (P.type & NGI_PRIM_D) ? P.D_R : P.G_R;
-            for (int k = 0; k < 3; ++k)
-                if (R[k] != 0) (*adj->gR)[3 * (size_t)r.first + k] += (adj->S[k] - r.second[k]) / R[k];
-        }
-    }
+    if (adj) adj_finish_path(*adj, s);
 }
 
 // One light path: ProcessSample_LT (direct = false, src/nanogi.cpp:804-934)
@@ -1288,6 +1314,23 @@ int oracle_render_adjoint(const ngi_scene_desc* d, const ngi_render_params* prm,
     const uint64_t npix = (uint64_t)prm->width * prm->height;
     const uint64_t total = prm->sampling == NGI_SAMPLING_PER_PIXEL ? npix * (uint64_t)prm->spp : (uint64_t)prm->num_samples;
     const size_t np = d->num_primitives;
+    // Replay scene: zero reflectance / radiance channels set to 1 (path
+    // sampling does not depend on them); the true values kept aside.
+    std::vector<double> Rt(3 * np, 0.0), Let(3 * np, 0.0);
+    for (size_t i = 0; i < np; ++i) {
+        ngi_primitive& P = s.prims[i].p;
+        double* R = (P.type & NGI_PRIM_D) ? P.D_R : ((P.type & NGI_PRIM_G) ? P.G_R : nullptr);
+        for (int k = 0; k < 3; ++k) {
+            if (R) {
+                Rt[3 * i + k] = R[k];
+                if (R[k] == 0.0) R[k] = 1.0;
+            }
+            if (P.type & NGI_PRIM_L) {
+                Let[3 * i + k] = P.Le[k];
+                if (P.Le[k] == 0.0) P.Le[k] = 1.0;
+            }
+        }
+    }
     int T = num_threads > 0 ? num_threads : (int)std::thread::hardware_concurrency();
     T = std::max(1, T);
     std::vector<std::vector<double>> R(T, std::vector<double>(3 * np, 0.0)), L(T, std::vector<double>(3 * np, 0.0));
@@ -1298,6 +1341,8 @@ int oracle_render_adjoint(const ngi_scene_desc* d, const ngi_render_params* prm,
         adj.scale = (double)npix / (double)total;
         adj.gR = &R[k];
         adj.gLe = &L[k];
+        adj.Rt = &Rt;
+        adj.Let = &Let;
         std::vector<double> dummy;
         Counts cnt;
         for (;;) {

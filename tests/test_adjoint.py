@@ -70,6 +70,66 @@ def test_oracle_adjoint_glossy_and_reference_sampling():
         assert abs(gR[prim, 1] - fd) <= 1e-6 * max(1.0, abs(fd))
 
 
+def _zero_some_parameters(scene, refl_zero, le_zero):
+    """Set the given (primitive, channel) reflectances and radiances to 0
+    (channel None = all three)."""
+    d = scene.desc()["primitives"]
+    refl = np.where((d["type"] & ngi.PRIM_D)[:, None] > 0, d["D_R"], d["G_R"]).copy()
+    Le = d["Le"].copy()
+    for prim, ch in refl_zero:
+        refl[prim, slice(None) if ch is None else ch] = 0.0
+    for prim, ch in le_zero:
+        Le[prim, slice(None) if ch is None else ch] = 0.0
+    scene.set_params(refl, Le)
+
+
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_oracle_adjoint_at_zero_parameters(renderer):
+    """A zero reflectance or radiance channel still has a derivative (each
+    contribution is linear in it): reflectance channel 0 of primitive 3, all
+    channels of primitive 5, and one radiance channel of the light at 0.
+    Central differences straddle 0 (the estimator is a polynomial in the
+    parameters and accepts negative values)."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    prims = s.desc()["primitives"]
+    light = int(np.nonzero(prims["type"] & ngi.PRIM_L)[0][0])
+    _zero_some_parameters(s, [(3, 0), (5, None)], [(light, 1)])
+    p = _params(renderer, spp=24)
+    g = np.random.default_rng(4).normal(size=(p.height, p.width, 3))
+    gR, gL = oracle.render_adjoint(s, p, g)
+    cases = [("R", 3, 0), ("R", 3, 1), ("R", 5, 0), ("R", 5, 2), ("R", 4, 1), ("Le", light, 1), ("Le", light, 0)]
+    for which, prim, ch in cases:
+        fd = _fd(s, p, g, oracle.render, which, prim, ch)
+        an = (gR if which == "R" else gL)[prim, ch]
+        assert abs(an - fd) <= 1e-6 * max(1.0, abs(fd)), (which, prim, ch, an, fd)
+    # The zero channels do get gradients.
+    assert abs(gR[3, 0]) > 1e-3 and abs(gR[5, 2]) > 1e-3 and abs(gL[light, 1]) > 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_gpu_adjoint_at_zero_parameters(renderer):
+    """The GPU replay at zero-valued parameters against its own central
+    differences (common random numbers) and the FP64 oracle's gradient."""
+    s = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
+    _zero_some_parameters(s, [(0, 1), (3, None)], [(7, 2)])
+    p = _params(renderer, 24, 24, 256, 8, 4)
+    p.engine = ngi.ENGINE_PERSISTENT
+    g = np.random.default_rng(5).normal(size=(24, 24, 3))
+    gR, gL, _ = ngi.render_adjoint(s, p, g)
+    oR, oL = oracle.render_adjoint(s, p, g)
+    render = lambda sc, q: ngi.render(sc, q)[0]
+    big = np.abs(np.concatenate([gR.ravel(), gL.ravel()])).max()
+    for which, prim, ch in [("R", 0, 1), ("R", 3, 0), ("R", 3, 2), ("R", 4, 1), ("Le", 7, 2), ("Le", 8, 0)]:
+        fd = _fd(s, p, g, render, which, prim, ch)
+        an = (gR if which == "R" else gL)[prim, ch]
+        assert abs(an - fd) <= 2e-3 * max(abs(fd), 1e-2 * big), (which, prim, ch, an, fd)
+    assert abs(gR[0, 1]) > 1e-3 * big and abs(gL[7, 2]) > 1e-3 * big
+    ref = np.concatenate([oR.ravel(), oL.ravel()])
+    mine = np.concatenate([gR.ravel(), gL.ravel()])
+    assert np.linalg.norm(mine - ref) <= 2e-2 * np.linalg.norm(ref)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
 def test_gpu_adjoint_matches_oracle_and_fd(renderer):

@@ -195,3 +195,94 @@ def test_batch_chunk_size_follows_the_paths_per_wave():
         assert np.abs(fb).max() > 0
         np.testing.assert_allclose(fa[i], fb, rtol=1e-9, atol=1e-12 * np.abs(fb).max())
     small.close()
+
+
+def test_set_params_waits_for_a_running_frame():
+    """Two batches over one scene: frame 0 of the first is still running when
+    the parameters change and the second batch renders (its refresh rewrites
+    the scene's parameter tables).  The rewrite waits for every batch stream
+    that reads the scene, so the first frame renders the old parameters
+    throughout and the second the new ones."""
+    def fresh():
+        return ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    s = fresh()
+    p = _p("ptdirect", 256, 256, 64, m=9, rr=-1, seed=12)
+    a = ngi.Batch([s], [p])
+    b = ngi.Batch([s], [p])
+    d = s.desc()["primitives"]
+    refl = np.where((d["type"] & ngi.PRIM_D)[:, None] > 0, d["D_R"], d["G_R"]).copy()
+    new_refl = np.where(refl > 0, 0.95, 0.0)
+    new_le = d["Le"] * 0.25
+    a.render(0)                       # enqueued, not waited for
+    s.set_params(new_refl, new_le)    # host copy only; uploaded by the next refresh
+    b.render(0)
+    a.sync()
+    b.sync()
+    old_ref, _ = ngi.render(fresh(), p)
+    s_new = fresh()
+    s_new.set_params(new_refl, new_le)
+    new_ref, _ = ngi.render(s_new, p)
+    _same(a.film(0), old_ref)
+    _same(b.film(0), new_ref)
+    a.close()
+    b.close()
+
+
+def test_session_survives_a_scene_rebuild():
+    """A scene rebuilt with another BVH releases its device copy; a session
+    of it keeps its film and renders on after uploading the scene again."""
+    s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    s.build(ngi.BVH_8)
+    p = _p("pt", 32, 32, 4, m=6, rr=-1, seed=2)
+    sess = ngi.Session(s, p)
+    sess.render(4)
+    before = sess.film()
+    s.build(ngi.BVH_2)
+    np.testing.assert_array_equal(sess.film(), before)
+    st = sess.stats()
+    assert st["samples"] == 32 * 32 * 4
+    sess.render(4)
+    both = sess.film()
+    q = _p("pt", 32, 32, 8, m=6, rr=-1, seed=2)
+    ref, _ = ngi.render(s, q)
+    _same(both, ref)
+    sess.reset()
+    assert not sess.film().any()
+    sess.close()
+
+
+def test_reduced_room_batch_against_the_oracle():
+    """The many-room workload's recipe (seeded log-uniform draws: 10-5,000
+    triangles, 64-512 px, 16-512 spp; ptdirect, at most 16 segments,
+    roulette from depth 4), first 16 rooms, images capped at 64 x 64 and
+    16 spp so the oracle finishes in seconds.  One captured-graph frame;
+    every film against oracle.render of its own room with the same seed and
+    stream id (common random numbers)."""
+    rng = np.random.default_rng(2024)
+    lu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    scenes, params = [], []
+    for i in range(16):
+        tris = int(round(lu(10, 5000)))
+        side = min(int(round(lu(64, 512))), 64)
+        spp = min(int(round(lu(16, 512))), 16)
+        s = ngi.Scene.generate(ngi.GEN_BATCH_ROOM, 1000 + i, 1.0, max(tris, 12))
+        s.build(8)
+        scenes.append(s)
+        p = ngi.default_params(renderer="ptdirect", width=side, height=side, sampling=ngi.SAMPLING_PER_PIXEL, spp=spp,
+                               max_num_vertices=17, rr_depth=4, rr_prob=0.5, seed=77, engine=ngi.ENGINE_PERSISTENT)
+        p.stream_id = i
+        params.append(p)
+    assert max(sc.num_triangles for sc in scenes) > 1000 and min(sc.num_triangles for sc in scenes) < 200
+    b = ngi.Batch(scenes, params)
+    assert b.config()["use_graph"]
+    b.render(0)
+    b.sync()
+    for i, (s, p) in enumerate(zip(scenes, params)):
+        f = b.film(i)
+        o = oracle.render(s, p)
+        err = np.abs(f - o).max(axis=2)
+        scale = np.abs(o).max(axis=2) + 1e-3 * np.abs(o).max()
+        frac = (err <= 2e-4 * scale).mean()
+        assert frac >= 0.98, (i, s.num_triangles, frac)
+        np.testing.assert_allclose(f.reshape(-1, 3).mean(0), o.reshape(-1, 3).mean(0), rtol=2e-3, err_msg=str(i))
+    b.close()

@@ -119,3 +119,32 @@ def test_empty_and_degenerate_inputs():
     assert h["tri"][0] != ngi.MISS and h["tri"][1] == ngi.MISS and h["tri"][2] != ngi.MISS
     # Axis-aligned rays (zero direction components) hit the walls at exact distances.
     assert h["t"][0] == pytest.approx(0.5, rel=1e-6) and h["t"][2] == pytest.approx(0.5, rel=1e-6)
+
+
+def test_trace_full_sphere_field_against_oracle_bvh():
+    """The benchmark's 1,833,314-triangle scene (displaced icosphere on a
+    512^2 heightfield), 10^6 rays: half from random points of the scene's
+    bounds in random directions, half aimed at random points of random
+    triangles (so most of them hit the displaced surface at grazing and
+    steep angles alike), against the oracle's FP64 BVH."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, 8)
+    assert s.num_triangles == 1833314
+    s.build(ngi.BVH_8)
+    d = s.desc()
+    P, F = d["positions"], d["faces"]
+    lo, hi = P.min(0), P.max(0)
+    rng = np.random.default_rng(11)
+    n = 500000
+    o1 = rng.uniform(lo, hi, (n, 3))
+    d1 = rng.normal(size=(n, 3))
+    o2 = rng.uniform(lo, hi, (n, 3))
+    tri = rng.integers(0, len(F), n)
+    b = rng.dirichlet((1.0, 1.0, 1.0), n)
+    tgt = (P[F[tri, 0]] * b[:, :1] + P[F[tri, 1]] * b[:, 1:2] + P[F[tri, 2]] * b[:, 2:])
+    d2 = tgt - o2
+    o = np.concatenate([o1, o2])
+    dirs = np.concatenate([d1, d2])
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = ngi.make_rays(o.astype(np.float32), dirs.astype(np.float32))
+    g, _ = _compare(s, rays, use_bvh=True)
+    assert (g["tri"] != ngi.MISS).mean() > 0.6
