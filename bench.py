@@ -4,10 +4,12 @@ One step = `--spp-per-step` samples per pixel over the whole image (paths of
 at most 32 segments, Russian roulette from depth 4), rendered into a batch
 film that is folded into per-pixel moments, so the RMSE of the estimate is
 measured from the batch variance.  The defaults (16 steps of 64 spp) render
-the full 1024-spp image of this workload.  Prints one JSON line (see README of the
-driver contract): samples/s, rays/s, the estimated RMSE at 1024 spp, the
-roofline of the dominant kernel (HBM-bound traversal) and the CPU FP64
-oracle timed on the host cores on a bounded sample of the same workload.
+the full 1024-spp image of this workload.  Prints one JSON line: samples/s,
+rays/s, the estimated RMSE at 1024 spp, the time each engine and precision
+needs to reach the RMSE of the FP64 estimator at 1024 spp against a
+16,384-spp FP64 reference (the metric's fixed-RMSE part), the roofline of
+the dominant kernel with its measured limiter, and the CPU FP64 oracle timed
+on the host cores on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -23,26 +25,61 @@ METRIC = "samples/sec at fixed RMSE vs FP64 ref (ptdirect, 1M tri, 1024²); adjo
 HBM_PEAK_GBS = 8000.0
 TRI_BYTES = 48      # one leaf triangle record (3 x float4)
 RAY_IO_BYTES = 48   # 32 B ray in + 16 B hit out
+SHQ_BYTES = 96      # shadow-queue record of the persistent kernel: 48 B written, 48 B read back
+FILM_BYTES = 48     # FP64 RGB read-modify-write of one contribution
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _profile_value(key):
+    """A `key: value` line of the committed rocprofv3 summary of the default
+    workload (profiles/persistent_ptdirect_sphere_field_1024.md), or None."""
+    path = os.path.join(ROOT, "profiles", "persistent_ptdirect_sphere_field_1024.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def is_default_workload(a, engine, precision):
+    import nanogi_amd as ngi
+    return (engine == ngi.ENGINE_PERSISTENT and precision == ngi.PRECISION_FP32_FP64ACC and a.bvh == 8 and
+            a.width == 1024 and a.height == 1024 and a.spp_per_step == 64 and a.detail == 8 and a.leaf == 0 and a.ct == 0)
+
+
 def measured_traffic(a, engine, precision):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
     WRITE_SIZE x 1.0, calibrated on a 1 GiB copy) for the default workload;
     None for other configurations."""
-    import nanogi_amd as ngi
-    path = os.path.join(ROOT, "profiles", "persistent_ptdirect_sphere_field_1024.md")
-    default = (engine == ngi.ENGINE_PERSISTENT and precision == ngi.PRECISION_FP32_FP64ACC and a.bvh == 8 and
-               a.width == 1024 and a.height == 1024 and a.spp_per_step == 64 and a.detail == 8 and a.leaf == 0 and a.ct == 0)
-    if not default or not os.path.exists(path):
+    if not is_default_workload(a, engine, precision):
         return None
-    for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+    v = _profile_value("hbm_bytes_per_launch")
+    return int(v) if v is not None else None
+
+
+def host_cpu_info():
+    """CPU model and counts of the host (/proc/cpuinfo and the scheduler)."""
+    model, sockets, cores = None, set(), set()
+    phys = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+                sockets.add(v)
+            elif k == "core id":
+                cores.add((phys, v))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(cores) or None,
+            "sockets": len(sockets) or None, "cpus_usable_by_this_process": len(os.sched_getaffinity(0))}
 
 
 def pct(xs, q):
@@ -235,9 +272,14 @@ def bench_adjoint(a):
 
 def rmse_study(a, scene, params, ngi):
     """RMSE of each estimator variant at `--rmse-spp` spp against an FP64
-    reference at `--rmse-ref-spp` spp (FP64 shading, persistent engine), the
-    per-pixel z-test against it, and the time each variant needs to reach
-    the FP64 estimator's RMSE (time scales with spp, error with 1/sqrt(spp))."""
+    reference at `--rmse-ref-spp` spp (FP64 shading, persistent engine), a
+    calibrated z-test against it, and the time each variant needs to reach
+    the FP64 estimator's RMSE (time scales with spp, error with 1/sqrt(spp)).
+
+    RMSE_target = the RMSE, net of the reference's own noise, of the FP64
+    estimator (persistent engine, FP64 shading) at `--rmse-spp` spp.  The
+    plain megakernel is the one-path-per-lane kernel with FP64 shading, on
+    the binary BVH and on the wide one."""
     import numpy as np
     W, H = a.width, a.height
     batch = 16   # spp per batch: the variants' variance comes from rmse_spp / 16 batch means
@@ -253,7 +295,7 @@ def rmse_study(a, scene, params, ngi):
         t0 = time.perf_counter()
         for k in range(spp // batch):
             se.render_batch(batch)
-            if (k + 1) % 32 == 0:
+            if (k + 1) % 64 == 0:
                 log(f"rmse study: {(k + 1) * batch}/{spp} spp")
         se.sync()
         dt = time.perf_counter() - t0
@@ -263,48 +305,62 @@ def rmse_study(a, scene, params, ngi):
 
     log(f"rmse study: FP64 reference at {a.rmse_ref_spp} spp")
     ref, ref_var, ref_s = run(ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL, a.rmse_ref_spp, 4242)
-    variants = [("persistent_fp32_fp64acc", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC),
-                ("persistent_fp32_all", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_ALL),
-                ("wavefront_fp32_fp64acc", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC),
-                ("wavefront_fp32_all", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_ALL),
-                ("megakernel_fp32_fp64acc", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC),
-                ("megakernel_fp64", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL),
-                ("persistent_fp64", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL)]
+    variants = [("persistent_fp32_fp64acc", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 8),
+                ("persistent_fp32_all", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_ALL, 8),
+                ("wavefront_fp32_fp64acc", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC, 8),
+                ("wavefront_fp32_all", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_ALL, 8),
+                ("megakernel_fp32_fp64acc", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC, 8),
+                ("persistent_fp64", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL, 8),
+                ("megakernel_fp64", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL, 8),
+                ("megakernel_bvh2_fp64", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL, 2)]
     out = {"reference": {"spp": a.rmse_ref_spp, "precision": "fp64 shading, fp32 traversal", "seconds": round(ref_s, 2),
                          "rmse_noise": float(np.sqrt(np.mean(ref_var)))}, "spp": a.rmse_spp, "variants": {}}
-    for name, eng, prec in variants:
-        film, var, dt = run(eng, prec, a.rmse_spp, 17)
+    bs = 8
+    blk = lambda x: x[:H // bs * bs, :W // bs * bs].reshape(H // bs, bs, W // bs, bs, 3).mean(axis=(1, 3))
+    for name, eng, prec, width in variants:
+        film, var, dt = run(eng, prec, a.rmse_spp, 17, width)
         err = film - ref
         rmse = float(np.sqrt(np.mean(err * err)))
         # Remove the reference's own noise from the measured error.
         rmse_vs_truth = float(np.sqrt(max(np.mean(err * err) - np.mean(ref_var), 0.0)))
         z = err / np.sqrt(var + ref_var + 1e-300)
         ok = (var + ref_var) > 0
-        # The same test on 8x8-pixel block means, whose distribution is closer
-        # to normal than single pixels' (paths that hit a light carry heavy tails).
-        # Pixel errors are independent between pixels, so block variances add.
-        bs = lambda x: x[:H // 8 * 8, :W // 8 * 8].reshape(H // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))
-        zb = bs(err) / np.sqrt(bs(var + ref_var) / 64.0 + 1e-300)
-        # |z| > 3 under a t distribution with (batches - 1) degrees of freedom
-        # (the variance comes from a finite number of batch means).
+        # Calibrated tests.  Block means over 8x8 pixels (single pixels are
+        # heavy-tailed: a few paths reach the small lights); pixel errors are
+        # independent, so block variances are mean variances / 64.  Under
+        # the null the block z-scores are close to N(0, 1) and sum(z^2) is
+        # chi-square with n degrees of freedom (reported as (chi2 - n) /
+        # sqrt(2 n) standard deviations).  A bias shared by the image shows
+        # in the z-score of the image mean, sum(err) / sqrt(sum(variances)),
+        # which does not suffer from the noise of per-block variance
+        # estimates.
+        zb = (blk(err) / np.sqrt((blk(var) + blk(ref_var)) / (bs * bs) + 1e-300)).ravel()
+        nb = zb.size
+        chi2 = float(np.sum(zb * zb))
+        z_img = float(np.sum(err) / np.sqrt(np.sum(var + ref_var) + 1e-300))
         from scipy import stats as _st
         dof = max(a.rmse_spp // batch - 1, 1)
         out["variants"][name] = {"rmse": rmse, "rmse_minus_ref_noise": rmse_vs_truth,
                                  "rmse_from_variance": float(np.sqrt(np.mean(var))),
                                  "seconds": round(dt, 3), "samples_per_s": round(a.rmse_spp * W * H / dt, 1),
+                                 "bvh_width": width,
                                  "z_gt3_frac": float(np.mean(np.abs(z[ok]) > 3.0)),
-                                 "z_gt3_frac_blocks8": float(np.mean(np.abs(zb) > 3.0)),
                                  "z_gt3_expected_t": float(2 * _st.t.sf(3.0, dof)), "batches": a.rmse_spp // batch,
+                                 "block_z": {"blocks": nb, "chi2_per_dof": round(chi2 / nb, 4),
+                                             "chi2_sigma": round((chi2 - nb) / math.sqrt(2 * nb), 2),
+                                             "image_mean_z": round(z_img, 3),
+                                             "gt3_frac": float(np.mean(np.abs(zb) > 3.0)), "gt3_expected": 0.0027},
                                  "mean_bias": float(np.mean(err)), "mean_rel_bias": float(np.mean(err) / np.mean(ref))}
-        log(f"rmse study: {name}: rmse {rmse:.5g} in {dt:.2f} s")
+        log(f"rmse study: {name}: rmse {rmse:.5g} in {dt:.2f} s, block chi2/dof {chi2 / nb:.3f}")
     target = out["variants"]["persistent_fp64"]["rmse_minus_ref_noise"]
     out["rmse_target"] = target
     for name, v in out["variants"].items():
         e = max(v["rmse_minus_ref_noise"], 1e-30)
         v["time_to_target_s"] = round(v["seconds"] * (e / target) ** 2, 3)
-    base = out["variants"]["megakernel_fp64"]["time_to_target_s"]
-    for name, v in out["variants"].items():
-        v["speedup_vs_megakernel_fp64"] = round(base / v["time_to_target_s"], 3)
+    for base_name in ("megakernel_fp64", "megakernel_bvh2_fp64"):
+        base = out["variants"][base_name]["time_to_target_s"]
+        for name, v in out["variants"].items():
+            v["speedup_vs_" + base_name] = round(base / v["time_to_target_s"], 3)
     if scene.bvh_info()["width"] != a.bvh:
         scene.build(a.bvh)
     return out
@@ -324,7 +380,7 @@ def bench_renderers(a):
     W = H = 256
     out = {"workload": "Cornell box (36 triangles), 256x256, 64 spp, max 8 segments, no Russian roulette"}
     ol = oracle.LoadedScene(scene)
-    threads = min(a.cpu_threads, os.cpu_count() or 1)
+    threads = max(1, min(a.cpu_threads, len(os.sched_getaffinity(0))))
     for r in ("pt", "ptdirect", "lt", "ltdirect", "bdpt"):
         p = ngi.default_params(renderer=r, width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=64, max_num_vertices=9,
                                rr_depth=-1, seed=1, engine=ngi.ENGINE_PERSISTENT)
@@ -399,7 +455,8 @@ def parse():
     ap.add_argument("--leaf", type=int, default=0, help="BVH maximum leaf size (0 = builder default)")
     ap.add_argument("--ct", type=float, default=0.0, help="BVH SAH node-visit cost (0 = builder default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle budget (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="CPU oracle threads: at most the CPUs this process may run on (the GPU box grants 16 per GPU)")
     ap.add_argument("--no-counters", action="store_true")
     ap.add_argument("--paths-in-flight", type=int, default=0,
                     help="wavefront queue capacity or persistent lanes (0 = auto)")
@@ -412,7 +469,7 @@ def parse():
     ap.add_argument("--nave-spp", type=int, default=256,
                     help="spp of the nave (ptdirect, 1024^2) engine comparison (0 = skip)")
     ap.add_argument("--renderers", type=int, default=1, help="time every renderer on the Cornell box (0 = skip)")
-    ap.add_argument("--rmse-ref-spp", type=int, default=0,
+    ap.add_argument("--rmse-ref-spp", type=int, default=16384,
                     help="render an FP64 reference at this spp and report each variant's RMSE against it (0 = skip)")
     ap.add_argument("--rmse-spp", type=int, default=1024, help="spp of each variant in the RMSE study")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -504,41 +561,62 @@ def main():
     rmse_1024 = rmse_now * math.sqrt(spp_done / 1024.0) if rmse_now is not None else None
     log(f"timed: {samples} paths in {elapsed:.3f} s -> {value / 1e6:.2f} M samples/s, {rays / elapsed / 1e9:.3f} G rays/s")
 
-    # Roofline of the dominant kernel.
+    # Roofline of the dominant kernel.  The unit is one ray query; the
+    # algorithmic bytes follow the traversal's own node-visit and
+    # triangle-test counts (128 B per wide node, 48 B per triangle), plus
+    # the memory the kernel's design moves per query.
     node_bytes = info["node_bytes"]
     roof = None
     if counts is not None:
-        if engine == ngi.ENGINE_PERSISTENT:
-            # One persistent launch per step: every path of the step.
-            avg_ms = st["ms_shade"] / a.steps
-            per_path = (counts["ext_rays_per_path"] * (node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]) +
-                        counts["shadow_rays_per_path"] * (node_bytes * counts["shadow_nodes_per_ray"] + TRI_BYTES * counts["shadow_tris_per_ray"]))
-            rays_per_launch = a.spp_per_step * npix
-            per_ray = per_path
-            kernel = "persistent path kernel (path regeneration)"
-        elif engine == ngi.ENGINE_WAVEFRONT and st["extend_launches"] > 0:
-            avg_ms = st["ms_extend"] / st["extend_launches"]
-            rays_per_launch = st["extend_rays_timed"] / st["extend_launches"]
-            per_ray = RAY_IO_BYTES + node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]
-            kernel = "extend (closest-hit traversal)"
+        er, sr = counts["ext_rays_per_path"], counts["shadow_rays_per_path"]
+        trav_per_path = (er * (node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]) +
+                         sr * (node_bytes * counts["shadow_nodes_per_ray"] + TRI_BYTES * counts["shadow_tris_per_ray"]))
+        if engine in (ngi.ENGINE_PERSISTENT, ngi.ENGINE_MEGAKERNEL):
+            # Path and ray state live in registers: no ray I/O or path-state
+            # bytes.  The persistent FP32 ptdirect kernel queues its shadow
+            # rays in HBM (one 48-B record written, read once); every traced
+            # shadow ray may add one FP64 RGB contribution (an upper bound:
+            # occluded ones add nothing).
+            shq = SHQ_BYTES * sr if (engine == ngi.ENGINE_PERSISTENT and precision != ngi.PRECISION_FP64_ALL) else 0.0
+            per_path = trav_per_path + shq + FILM_BYTES * sr
+            if engine == ngi.ENGINE_PERSISTENT:
+                avg_ms = st["ms_shade"] / a.steps          # one launch per step
+                paths_per_launch = a.spp_per_step * npix
+                kernel = "persistent path kernel (path regeneration)"
+            else:
+                mk_launches = max(1, math.ceil(samples / max(1, (1 << 22) // npix * npix)))
+                avg_ms = st["ms_shade"] / mk_launches
+                paths_per_launch = samples / mk_launches
+                kernel = "megakernel (one path per lane)"
+            units = paths_per_launch * (er + sr)
+            bytes_per_launch = per_path * paths_per_launch
+            excluded = "ray in/out and path state (registers in this kernel)"
         else:
-            launches = max(1, st["trace_launches"] or (st["extend_launches"] or 0))
-            # Megakernel: all rays of a launch; ray state stays in registers.
-            mk_launches = max(1, math.ceil(samples / max(1, (1 << 22) // npix * npix)))
-            avg_ms = st["ms_shade"] / mk_launches
-            paths_per_launch = samples / mk_launches
-            per_path = (counts["ext_rays_per_path"] * (node_bytes * counts["ext_nodes_per_ray"] + TRI_BYTES * counts["ext_tris_per_ray"]) +
-                        counts["shadow_rays_per_path"] * (node_bytes * counts["shadow_nodes_per_ray"] + TRI_BYTES * counts["shadow_tris_per_ray"]))
-            rays_per_launch = paths_per_launch
-            per_ray = per_path
-            kernel = "megakernel (one path per lane)"
-            del launches
-        bytes_per_launch = per_ray * rays_per_launch
+            # Wavefront extension stage: rays in and hits out through HBM.
+            avg_ms = st["ms_extend"] / max(1, st["extend_launches"])
+            units = st["extend_rays_timed"] / max(1, st["extend_launches"])
+            bytes_per_launch = units * (RAY_IO_BYTES + node_bytes * counts["ext_nodes_per_ray"] +
+                                        TRI_BYTES * counts["ext_tris_per_ray"])
+            kernel = "extend (closest-hit traversal)"
+            excluded = "none"
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(a, engine, precision)
         roof = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(a, engine, precision),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(bytes_per_launch),
-                "units_per_launch": round(rays_per_launch), "bytes_per_unit": round(per_ray, 2)}
+                "unit_of_work": "ray query (extension + shadow)", "units_per_launch": round(units),
+                "bytes_per_unit": round(bytes_per_launch / max(units, 1.0), 2), "excluded_bytes": excluded}
+        if traffic is not None:
+            roof["traffic_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if is_default_workload(a, engine, precision):
+            # What limits the kernel (rocprofv3 --pmc on this workload): the
+            # share of wave cycles spent waiting on dependencies and the
+            # share of lanes active per VALU instruction.
+            wait, valu = _profile_value("sq_wait_any_frac"), _profile_value("valu_utilization")
+            if wait is not None and valu is not None:
+                roof["limiter"] = "latency of dependent traversal loads and SIMD divergence, not HBM bandwidth"
+                roof["sq_wait_any_frac"] = wait
+                roof["valu_utilization"] = valu
 
     # The other engines on the same workload (short runs, same seeds).
     engines = {}
@@ -546,6 +624,8 @@ def main():
         variants = [("megakernel_bvh8_fp32", ngi.ENGINE_MEGAKERNEL, 8, ngi.PRECISION_FP32_FP64ACC),
                     ("wavefront_bvh8_fp32", ngi.ENGINE_WAVEFRONT, 8, ngi.PRECISION_FP32_FP64ACC),
                     ("persistent_bvh8_fp32", ngi.ENGINE_PERSISTENT, 8, ngi.PRECISION_FP32_FP64ACC),
+                    ("persistent_bvh8_fp64", ngi.ENGINE_PERSISTENT, 8, ngi.PRECISION_FP64_ALL),
+                    ("megakernel_bvh8_fp64", ngi.ENGINE_MEGAKERNEL, 8, ngi.PRECISION_FP64_ALL),
                     ("megakernel_bvh2_fp64", ngi.ENGINE_MEGAKERNEL, 2, ngi.PRECISION_FP64_ALL)]
         for name, eng, width, prec in variants:
             if eng == engine and width == a.bvh and prec == precision:
@@ -584,27 +664,36 @@ def main():
     if a.cpu_seconds > 0:
         try:
             import oracle
-            threads = min(a.cpu_threads, os.cpu_count() or 1)
+            threads = max(1, min(a.cpu_threads, len(os.sched_getaffinity(0))))
             ol = oracle.LoadedScene(scene)
             p = params()
+
+            def timed(first, count, nthreads):
+                t1 = time.perf_counter()
+                _, cc = ol.render_paths(p, first, count, nthreads)
+                return time.perf_counter() - t1, cc
+
             # Calibrate on growing prefixes until one takes >= 1 s.
             n, dt = 4096, 0.0
             while True:
-                t1 = time.perf_counter()
-                ol.render_paths(p, 0, n, threads)
-                dt = time.perf_counter() - t1
+                dt, _ = timed(0, n, threads)
                 if dt >= 1.0 or n >= npix:
                     break
                 n = min(npix, n * 4)
             n2 = int(min(max(n, n * a.cpu_seconds / max(dt, 1e-3)), 64 * npix))
-            t1 = time.perf_counter()
-            _, cc = ol.render_paths(p, n, n2, threads)
-            dt = time.perf_counter() - t1
+            dt, cc = timed(n, n2, threads)
+            # One thread on a sample of about a third of the budget.
+            n1 = int(max(4096, min(n2, n2 / threads * a.cpu_seconds / 3.0 / max(dt, 1e-3))))
+            dt1, cc1 = timed(n + n2, n1, 1)
             cpu = {"value": round(n2 / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
                    "sample": f"{n2} path samples (path ids {n}..{n + n2 - 1}; pixel = id mod W*H, sample = id div W*H) "
-                             f"of the same ptdirect workload; FP64 CPU restatement with its own BVH, {threads} threads",
-                   "rays_per_s": round(float(cc.sum()) / dt, 1)}
-            log(f"cpu oracle: {n2} paths in {dt:.2f} s on {threads} threads")
+                             f"of the same ptdirect workload; FP64 CPU restatement of the reference estimator with its own "
+                             f"median-split BVH (not the reference binary, which needs Embree 2 and TBB), {threads} threads",
+                   "rays_per_s": round(float(cc.sum()) / dt, 1),
+                   "one_thread": {"value": round(n1 / dt1, 1), "rays_per_s": round(float(cc1.sum()) / dt1, 1),
+                                  "sample": f"{n1} path samples (path ids {n + n2}..{n + n2 + n1 - 1})"},
+                   "host": host_cpu_info()}
+            log(f"cpu oracle: {n2} paths in {dt:.2f} s on {threads} threads; {n1} paths in {dt1:.2f} s on 1 thread")
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {e}")
 
@@ -620,6 +709,31 @@ def main():
     if a.adjoint_steps > 0:
         adjoint = bench_adjoint(a)
 
+    # The metric's fixed-RMSE part: time for the timed engine and precision
+    # to reach RMSE_target, against the plain FP64 megakernels.
+    fixed_rmse = None
+    if study is not None:
+        key = {(ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC): "persistent_fp32_fp64acc",
+               (ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_ALL): "persistent_fp32_all",
+               (ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL): "persistent_fp64",
+               (ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC): "wavefront_fp32_fp64acc",
+               (ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_ALL): "wavefront_fp32_all",
+               (ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC): "megakernel_fp32_fp64acc",
+               (ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL): "megakernel_fp64"}.get((engine, precision))
+        if key is not None:
+            v = study["variants"][key]
+            # At the timed run's rate (64 spp per launch), which the study's
+            # 16-spp batches do not reach.
+            t_main = v["time_to_target_s"] * v["samples_per_s"] / value
+            fixed_rmse = {"variant": key, "rmse_target": study["rmse_target"],
+                          "rmse_target_definition": f"RMSE of the FP64 estimator (persistent engine, FP64 shading) at "
+                                                    f"{a.rmse_spp} spp against a {a.rmse_ref_spp}-spp FP64 reference, "
+                                                    f"net of the reference's own noise",
+                          "time_to_target_s": round(t_main, 3), "time_to_target_s_in_study": v["time_to_target_s"],
+                          "speedup_vs_plain_megakernel_bvh2_fp64": round(study["variants"]["megakernel_bvh2_fp64"]["time_to_target_s"] / t_main, 3),
+                          "speedup_vs_plain_megakernel_bvh8_fp64": round(study["variants"]["megakernel_fp64"]["time_to_target_s"] / t_main, 3),
+                          "block_z": v["block_z"]}
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(elapsed * 1e3 / a.steps, 3), "higher_is_better": True,
@@ -632,6 +746,7 @@ def main():
                    "engine": a.engine, "precision": a.precision, "bvh_width": a.bvh, "spp_per_step": a.spp_per_step,
                    "triangles": ntris},
         "rays_per_s": round(rays / elapsed, 1),
+        "fixed_rmse": fixed_rmse,
         "rmse_estimate": {"spp": spp_done, "rmse": rmse_now, "rmse_at_1024spp": rmse_1024,
                           "time_to_1024spp_s": round(1024 * npix / value, 3)},
         "roofline": roof,

@@ -174,10 +174,10 @@ def test_independent_seed_block_z_test(renderer):
     limit theorem holds where single pixels are heavy-tailed).  With each
     side's variance from its own batch means, z = (G - O) / sqrt(VG + VO)
     is close to N(0, 1) under the null.  Bounds: sum(z^2) < 48 (chi-square,
-    12 degrees of freedom: P ~ 1e-5), max |z| < 4.5, and the mean of z
-    times sqrt(12) within 4.5 (a bias common to all blocks; a 1% bias gives
-    about 20 for ptdirect here).  Twelve oracle-against-oracle trials of
-    this recipe gave at most 34 / 3.2 / 3.0."""
+    12 degrees of freedom: P ~ 1e-5), max |z| < 4.5, and the z-score of the
+    image mean, sum(G - O) / sqrt(sum(VG + VO)), within 4.5 (a bias common
+    to all blocks; a 1% bias gives about 20 for ptdirect here).  Twelve
+    oracle-against-oracle trials of this recipe gave at most 34 / 3.2 / 3.0."""
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
     W = H = 16
     p = _params(renderer, W, H, 1, 9, rr_depth=3, engine=ngi.ENGINE_PERSISTENT)
@@ -196,7 +196,8 @@ def test_independent_seed_block_z_test(renderer):
     chi2 = float(np.sum(z * z))
     assert 1.0 < chi2 < 48.0, f"chi-square {chi2:.1f} over {z.size} block means"
     assert np.abs(z).max() < 4.5, np.abs(z).max()
-    assert abs(z.mean()) * np.sqrt(z.size) < 4.5, z.mean()
+    z_img = float(np.sum(G - O) / np.sqrt(np.sum(VG + VO)))
+    assert abs(z_img) < 4.5, z_img
 
 
 def test_wavefront_small_queue_many_iterations():

@@ -4,7 +4,13 @@ Parity bar: identical triangle ids except at ties (hits within a relative
 1e-5 of each other in t, i.e. shared edges/vertices where either triangle is
 a correct answer; Embree's own tie-breaking is not pinned by the reference),
 identical hit/miss except for grazing rays (<= 0.05%), and t within
-relative 1e-5 (FP32 traversal vs FP64).
+relative 1e-5 (FP32 traversal vs FP64).  A different triangle at a
+different distance is an edge case only where the ray passes an edge of the
+triangle the two disagree on (the oracle's, if the GPU went past it; the
+GPU's, if the oracle missed it) closer than one FP32 ulp of the coordinate
+magnitude there: the FP32 test then rounds the ray to the other side of
+the edge (grazing rays on finely tessellated surfaces), and at most 1e-4 of
+the rays may be such cases.
 """
 import numpy as np
 import pytest
@@ -30,6 +36,47 @@ def _bounds(scene):
     return p.min(0), p.max(0)
 
 
+def _ray_edge_distance(o, d, tri_pts):
+    """FP64 distance from each ray (o + s d, s >= 0; rows) to the nearest
+    edge segment of its triangle (tri_pts: n x 3 x 3), and the ray
+    parameter there."""
+    o = o.astype(np.float64)
+    d = d.astype(np.float64)
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    best = np.full(len(o), np.inf)
+    best_s = np.zeros(len(o))
+    for a_i, b_i in ((0, 1), (1, 2), (2, 0)):
+        a, b = tri_pts[:, a_i], tri_pts[:, b_i]
+        e = b - a
+        ee = np.einsum("ij,ij->i", e, e)
+        w = o - a
+        de, dw, ew = np.einsum("ij,ij->i", d, e), np.einsum("ij,ij->i", d, w), np.einsum("ij,ij->i", e, w)
+        den = ee - de * de
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sc = np.where(den > 1e-300, (de * ew - ee * dw) / den, 0.0)
+        sc = np.maximum(sc, 0.0)
+        tc = np.clip(np.einsum("ij,ij->i", o + sc[:, None] * d - a, e) / ee, 0.0, 1.0)
+        sc = np.maximum(np.einsum("ij,ij->i", a + tc[:, None] * e - o, d), 0.0)
+        tc = np.clip(np.einsum("ij,ij->i", o + sc[:, None] * d - a, e) / ee, 0.0, 1.0)
+        dist = np.linalg.norm(o + sc[:, None] * d - (a + tc[:, None] * e), axis=1)
+        better = dist < best
+        best = np.where(better, dist, best)
+        best_s = np.where(better, sc, best_s)
+    return best, best_s
+
+
+def _edge_cases(scene, rays, g, o, idx):
+    """Rays idx (different triangle at a different distance) whose disputed
+    triangle has an edge within one FP32 ulp of the coordinate magnitude."""
+    d = scene.desc()
+    P, F = d["positions"], d["faces"]
+    disputed = np.where(g["t"][idx] > o["t"][idx], o["tri"][idx], g["tri"][idx])
+    pts = P[F[disputed]]
+    dist, s_at = _ray_edge_distance(rays["o"][idx], rays["d"][idx], pts)
+    mag = np.abs(rays["o"][idx]).max(axis=1).astype(np.float64) + s_at
+    return dist <= mag * 2.0 ** -24
+
+
 def _compare(scene, rays, use_bvh, any_hit=False):
     g = scene.trace(rays, any_hit=any_hit)
     o = oracle.trace(scene, rays, any_hit=any_hit, use_bvh=use_bvh)
@@ -44,8 +91,13 @@ def _compare(scene, rays, use_bvh, any_hit=False):
     close = np.abs(tg - to) <= T_RTOL * np.maximum(1.0, np.abs(to))
     assert close.mean() >= 0.9995, f"t mismatch on {(~close).sum()} rays"
     same = g["tri"][both] == o["tri"][both]
-    # Differing ids are only allowed where the two hits coincide (a tie).
-    assert np.all(same | close), "different triangle at a different distance"
+    # Differing ids are allowed where the two hits coincide (a tie), or where
+    # the ray passes an edge of the disputed triangle within FP32 rounding.
+    idx = np.nonzero(both)[0][~(same | close)]
+    if len(idx):
+        assert len(idx) <= max(1, int(1e-4 * n)), f"{len(idx)} rays hit a different triangle at a different distance"
+        edge = _edge_cases(scene, rays, g, o, idx)
+        assert edge.all(), f"rays {idx[~edge][:8]} hit a different triangle at a different distance, away from any edge"
     # Coplanar overlaps (box bottoms on the floor) make genuine ties common.
     assert same.mean() >= 0.98, f"triangle id agreement {same.mean():.5f}"
     return g, o
