@@ -273,6 +273,25 @@ typedef struct ngi_stats {
  * src/nanogi.cpp:429-437).  stats may be NULL. */
 int ngi_render(ngi_scene* scene, const ngi_render_params* p, double* film_out, ngi_stats* stats);
 
+/* Render without allocating device memory.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under stream or
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Renders *p like ngi_render into the caller's device buffer d_film
+ * (W*H*3 doubles, or floats with NGI_PRECISION_FP32_ALL), already scaled by
+ * W*H/N, enqueued on the scene's stream.  Counters, the path counter and
+ * the shadow-ray queues live in one static device array of 30 MiB per GPU,
+ * held until the process ends; the lanes in flight are capped at 327,680
+ * (5,120 wavefront queues), and a larger paths_in_flight is rejected with
+ * NGI_ERR_INVALID_ARG.  pt, ptdirect, lt and ltdirect on the persistent
+ * engine or the megakernel; the wavefront engine and bdpt are rejected with
+ * NGI_ERR_UNSUPPORTED.  The scene must already be resident with its current
+ * parameters (after ngi_render / ngi_trace / a session render since the
+ * last build or set_params), else NGI_ERR_INVALID_ARG.  stats NULL: no
+ * synchronisation; non-NULL: waits for the render and fills the counts.
+ * Every call on one GPU uses the same array: calls that overlap on one GPU
+ * (other scenes' streams, other threads) race, so serialise them. */
+int ngi_render_device_noalloc(ngi_scene* scene, const ngi_render_params* p, void* d_film, ngi_stats* stats);
+
 /* Incremental rendering (the reference's time-budget chunks,
  * src/nanogi.cpp:243-418). */
 typedef struct ngi_session ngi_session;

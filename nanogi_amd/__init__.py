@@ -116,7 +116,7 @@ EXPORTS = [
     "ngi_last_error", "ngi_device_count", "ngi_scene_load_yaml", "ngi_scene_create", "ngi_scene_generate",
     "ngi_scene_save_yaml", "ngi_scene_destroy", "ngi_scene_get_desc", "ngi_scene_set_params", "ngi_scene_build",
     "ngi_scene_bvh_info", "ngi_scene_bvh_export", "ngi_trace", "ngi_trace_device", "ngi_renderer_from_string",
-    "ngi_render_params_default", "ngi_render", "ngi_session_create", "ngi_session_render",
+    "ngi_render_params_default", "ngi_render", "ngi_render_device_noalloc", "ngi_session_create", "ngi_session_render",
     "ngi_session_render_batch", "ngi_session_sync", "ngi_session_get_film", "ngi_session_get_stats",
     "ngi_session_reset", "ngi_session_destroy", "ngi_render_adjoint", "ngi_render_batch", "ngi_image_save",
     "ngi_batch_opts_default", "ngi_batch_create", "ngi_batch_render", "ngi_batch_sync", "ngi_batch_get_film",
@@ -162,6 +162,7 @@ def lib():
         "ngi_renderer_from_string": ([ctypes.c_char_p], c_int),
         "ngi_render_params_default": ([P(RenderParams)], None),
         "ngi_render": ([vp, P(RenderParams), vp, P(Stats)], c_int),
+        "ngi_render_device_noalloc": ([vp, P(RenderParams), vp, P(Stats)], c_int),
         "ngi_session_create": ([vp, P(RenderParams), P(vp)], c_int),
         "ngi_session_render": ([vp, c_int], c_int),
         "ngi_session_render_batch": ([vp, c_int], c_int),

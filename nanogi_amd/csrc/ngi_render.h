@@ -56,9 +56,11 @@ struct LaneMem {
     void** ptr;
     size_t* bytes;
     hipStream_t stream;
+    bool fixed = false;   // caller-owned buffer of *bytes bytes: never reallocated
     int reserve(size_t n) const {
         if (n == 0) return fail(NGI_ERR_INVALID_ARG, "empty per-lane record buffer");
         if (*bytes >= n) return NGI_OK;
+        if (fixed) return fail(NGI_ERR_INVALID_ARG, "launch needs %zu bytes of per-lane records, the fixed buffer holds %zu", n, *bytes);
         if (*ptr) {
             NGI_HIP_CHECK(hipStreamSynchronize(stream));
             NGI_HIP_CHECK(hipFree(*ptr));
@@ -89,6 +91,10 @@ int launch_bdpt(const RenderArgs& a, unsigned long long* next, LaneMem mem, bool
                 Lanes ln = Lanes());
 // Renderers that always run on the persistent engine.
 inline bool persistent_only(int renderer) { return is_light_tracer(renderer) || renderer == NGI_RENDERER_BDPT; }
+
+// Validation shared by every render entry point (renderer, sizes, sampling,
+// precision, engine, roulette, vertex cap, sensor).
+int check_render_params(const ngi_scene* sc, const ngi_render_params& p);
 
 struct Session {
     ngi_scene* scene = nullptr;

@@ -206,9 +206,7 @@ int Session::zero(void* p, size_t n_elems, bool f64) {
     return NGI_OK;
 }
 
-int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
-    scene = sc;
-    p = pp;
+int check_render_params(const ngi_scene* sc, const ngi_render_params& p) {
     if (p.renderer != NGI_RENDERER_PT && p.renderer != NGI_RENDERER_PTDIRECT && !is_light_tracer(p.renderer) &&
         p.renderer != NGI_RENDERER_BDPT)
         return fail(NGI_ERR_UNSUPPORTED, "renderer %d is not supported (pt, ptdirect, lt, ltdirect and bdpt are)", p.renderer);
@@ -224,7 +222,15 @@ int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
     if (p.max_num_vertices < -1) return fail(NGI_ERR_INVALID_ARG, "bad max_num_vertices");
     if (sc->sensor_prim < 0) return fail(NGI_ERR_INVALID_SCENE, "scene has no sensor (E) primitive");
     if (sc->prims[sc->sensor_prim].sensor_type != NGI_SENSOR_PINHOLE) return fail(NGI_ERR_UNSUPPORTED, "only pinhole sensors are supported");
-    int st = device_prepare(*sc);
+    return NGI_OK;
+}
+
+int Session::init(ngi_scene* sc, const ngi_render_params& pp) {
+    scene = sc;
+    p = pp;
+    int st = check_render_params(sc, p);
+    if (st) return st;
+    st = device_prepare(*sc);
     if (st) return st;
     film64 = p.precision != NGI_PRECISION_FP32_ALL;
     npix = (uint32_t)p.width * (uint32_t)p.height;
