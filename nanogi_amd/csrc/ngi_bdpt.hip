@@ -21,7 +21,12 @@
 // one vertex at a time, p_{i+1} / p_i = pL(x_i) / pE(x_i) with pL / pE the
 // area-measure pdfs of generating vertex x_i from the light / eye side; the
 // terms with c(i) = 0 are left out as in the reference (p_i = 0 there).  The
-// ratios are accumulated in FP64.
+// ratios are accumulated in FP64.  Away from the connection every factor of
+// the walk depends on one subpath only: after a sample's subpaths are drawn,
+// each vertex record gets its generation pdf from the previous vertex, its
+// pdf from the next one and whether the split before it has c != 0, so a
+// strategy evaluates BSDFs only at the two vertices on either side of its
+// connection (six pdf / c values) and reads the rest from the records.
 //
 // Random numbers: the light subpath's step k draws stream 1 at vertex k
 // (step 0: light choice and position, as the light tracers), the eye
@@ -58,6 +63,18 @@ struct BVert {
     bool deg;   // pinhole or point light
 };
 
+// Record of subpath vertex v_k (4 float4): (p, prim), (gn, type bits |
+// 256 degenerate | 512 split flag), (n, fwd), (alpha, rev), with
+//   fwd  = area pdf of generating v_k from v_{k-1} (incoming direction from
+//          v_{k-2}); v_0: position pdf;
+//   rev  = area pdf of generating v_k from v_{k+1} (incoming direction from
+//          v_{k+2} where the subpath has it);
+//   split flag = c != 0 for the split between the two vertices named below,
+//          with their outer neighbours on the same subpath: light side, v_k
+//          the split (v_{k-1}, v_k) (k = 0: v_0 emits towards v_1); eye side,
+//          the split (v_{k+1}, v_k).
+#define NGI_BD_SPLIT 512
+
 struct BdptMem {
     float4* v;       // 4 float4 per (side, vertex, lane), [side][vertex][lane]
     uint32_t lanes;  // grid * block
@@ -90,6 +107,31 @@ __device__ __forceinline__ BVert bload_geo(const BdptMem& m, int side, int k, ui
     v.n = f3(c.x, c.y, c.z);
     v.alpha = f3(0.0f);
     return v;
+}
+
+// The strategy-independent MIS factors of vertex k (see the record layout).
+struct BFac {
+    float fwd, rev;
+    bool split;
+};
+
+__device__ __forceinline__ BFac bload_fac(const BdptMem& m, int side, int k, uint32_t lane) {
+    const float4* q = brec(m, side, k, lane);
+    BFac f;
+    f.split = (__float_as_int(q[1].w) & NGI_BD_SPLIT) != 0;
+    f.fwd = q[2].w;
+    f.rev = q[3].w;
+    return f;
+}
+
+__device__ __forceinline__ void bstore_fac(const BdptMem& m, int side, int k, uint32_t lane, const BVert& v, float fwd, float rev,
+                                           bool split) {
+    float4* q = brec(m, side, k, lane);
+    q[1] = make_float4(v.gn.x, v.gn.y, v.gn.z, __int_as_float(v.type | (v.deg ? 256 : 0) | (split ? NGI_BD_SPLIT : 0)));
+    q[2] = make_float4(v.n.x, v.n.y, v.n.z, fwd);
+    float4 a = q[3];
+    a.w = rev;
+    q[3] = a;
 }
 
 __device__ __forceinline__ f3 bload_pos(const BdptMem& m, int side, int k, uint32_t lane) {
@@ -258,72 +300,117 @@ __device__ __forceinline__ f3 cst_at(const BdptCtx& c, int i, int n, f3 vm2, con
     return fsL * fsE * gterm(vL, vE);
 }
 
-// mis_weight() below with each vertex of the connected path loaded once
-// per walk: a window of three vertices (and the positions one further out)
-// slides away from the connection, and every pdf and c_i is evaluated from
-// it with the same arguments, hence the same values, as the definitions
-// above.
-__device__ __forceinline__ double mis_weight_window(const BdptCtx& c, const Conn& q) {
-    const int s = q.s, n = q.n;
-    double inv = 1.0, r = 1.0;
-    if (s >= 1) {
-        // Towards the light, i = s - 1 .. 0: B = x_i, C = x_{i+1}, pC2 = x_{i+2}.p.
-        BVert B = q.xg(s - 1), C = q.xg(s);
-        f3 pC2 = s + 1 <= n - 1 ? q.xp(s + 1) : f3(0.0f);
-        for (int i = s - 1; i >= 0; --i) {
-            // pdf_eye_side(i): i <= n - 2 here (t >= 1).
-            const f3 wiC = i + 2 <= n - 1 ? vnorm(pC2 - C.p) : f3(0.0f);
-            const double pe = (double)vert_pdf(c, C, i + 2 <= n - 1, wiC, dirv(C, B)) * (double)gterm(C, B);
-            if (pe == 0.0) break;
-            double pl;
-            f3 ci;
-            BVert A;
-            if (i == 0) {
-                pl = (double)pos_pdf(c, B);
-                ci = vert_eval<false>(c, B, false, f3(0.0f), dirv(B, C)) * pos_eval_unforced(c, B);
+// p * G as stored in the records (one rounding of the FP64 product).
+__device__ __forceinline__ float pdf_area(float pdf, float g) { return (float)((double)pdf * (double)g); }
+
+// The split flag of cst_at() (c != 0 for the split vL | vE; vm2 = the
+// outer neighbour of vL, vp1 = that of vE, each where it exists).
+__device__ __forceinline__ bool split_nonzero(const BdptCtx& c, bool has_m2, f3 vm2, const BVert& vL, const BVert& vE, bool has_p1,
+                                              f3 vp1) {
+    const f3 wiL = has_m2 ? vnorm(vm2 - vL.p) : f3(0.0f);
+    const f3 wiE = has_p1 ? vnorm(vp1 - vE.p) : f3(0.0f);
+    const f3 fsL = vert_eval<true>(c, vL, has_m2, wiL, dirv(vL, vE));
+    if (is_zero(fsL)) return false;
+    const f3 fsE = vert_eval<false>(c, vE, has_p1, wiE, dirv(vE, vL));
+    return !is_zero(fsL * fsE * gterm(vL, vE));
+}
+
+// Fills the MIS factors (fwd, rev, split flag) of the m vertices of one of
+// this lane's subpaths.  A window of three vertices A = v_{k-1}, B = v_k,
+// C = v_{k+1} and the positions v_{k-2}, v_{k+2} slides along the subpath.
+__device__ __forceinline__ void subpath_factors(const BdptCtx& c, const BdptMem& mem, int side, int m, uint32_t gl) {
+    if (m <= 0) return;
+    BVert A, B = bload_geo(mem, side, 0, gl), C;
+    f3 pA2(0.0f), pC2(0.0f);
+    if (m > 1) C = bload_geo(mem, side, 1, gl);
+    for (int k = 0; k < m; ++k) {
+        if (k + 2 <= m - 1) pC2 = bload_pos(mem, side, k + 2, gl);
+        // fwd: v_k generated from A = v_{k-1} (incoming from v_{k-2}).
+        const float fwd = k == 0 ? pos_pdf(c, B)
+                                 : pdf_area(vert_pdf(c, A, k >= 2, k >= 2 ? vnorm(pA2 - A.p) : f3(0.0f), dirv(A, B)), gterm(A, B));
+        // rev: v_k generated from C = v_{k+1} (incoming from v_{k+2}).
+        const float rev = k + 1 <= m - 1 ? pdf_area(vert_pdf(c, C, k + 2 <= m - 1, k + 2 <= m - 1 ? vnorm(pC2 - C.p) : f3(0.0f), dirv(C, B)),
+                                                    gterm(C, B))
+                                         : 0.0f;
+        bool split = false;
+        if (side == 0) {
+            // Split (v_{k-1}, v_k) with v_{k+1} beyond v_k; k = 0: emission of v_0 towards v_1.
+            if (k == 0) split = m > 1 && !is_zero(vert_eval<false>(c, B, false, f3(0.0f), dirv(B, C)) * pos_eval_unforced(c, B));
+            else if (k + 1 <= m - 1) split = split_nonzero(c, k >= 2, pA2, A, B, true, C.p);
+        } else if (k + 2 <= m - 1) {
+            // Split (v_{k+1}, v_k): v_{k+1} on the light side of the path, v_{k+2} beyond it, v_{k-1} beyond v_k.
+            split = split_nonzero(c, true, pC2, C, B, k >= 1, A.p);
+        }
+        bstore_fac(mem, side, k, gl, B, fwd, rev, split);
+        pA2 = A.p;
+        A = B;
+        B = C;
+        if (k + 2 <= m - 1) C = bload_geo(mem, side, k + 2, gl);
+    }
+}
+
+// mis_weight() below from the records' factors: only the factors next to
+// the connection (pE of x_{s-1} and x_{s-2}, pL of x_s and x_{s+1}, c_{s-1}
+// and c_{s+1}) depend on both subpaths and are evaluated here, with the same
+// arguments, hence the same values, as the definitions above.  Light
+// vertices y_k = x_k, eye vertices z_j = x_{n-1-j}.
+__device__ __forceinline__ double mis_weight_fac(const BdptCtx& c, const BdptMem& mem, uint32_t owner, int s, int n) {
+    const int t = n - s;
+    Conn q{&mem, owner, s, n};
+    // The six connection-dependent factors, with at most three vertices
+    // loaded at a time: pe1 = pE(x_{s-1}), pe2 = pE(x_{s-2}), pl0 = pL(x_s),
+    // pl1 = pL(x_{s+1}), c_{s-1} and c_{s+1}.
+    double pe1 = 0.0, pe2 = 0.0, pl0 = 0.0, pl1 = 0.0;
+    bool cm1 = false, cp1 = false;
+    {
+        const BVert E0 = q.xg(s);                               // x_s = z_{t-1} (a light when s = 0)
+        BVert L1;
+        f3 pL2(0.0f);
+        if (s >= 1) L1 = q.xg(s - 1);                           // x_{s-1} = y_{s-1}
+        if (s >= 2) pL2 = q.xp(s - 2);
+        if (s + 1 <= n - 1) {
+            const BVert E1 = q.xg(s + 1);                       // x_{s+1} = z_{t-2}
+            if (s >= 1) pe1 = (double)vert_pdf(c, E0, true, dirv(E0, E1), dirv(E0, L1)) * (double)gterm(E0, L1);
+            if (s + 2 <= n - 1) pl1 = (double)vert_pdf(c, E0, s >= 1, s >= 1 ? dirv(E0, L1) : f3(0.0f), dirv(E0, E1)) * (double)gterm(E0, E1);
+            const bool hp = s + 2 <= n - 1;
+            const f3 pp = hp ? q.xp(s + 2) : f3(0.0f);
+            cp1 = split_nonzero(c, s >= 1, s >= 1 ? L1.p : f3(0.0f), E0, E1, hp, pp);
+        } else if (s >= 1) {
+            pe1 = (double)vert_pdf(c, E0, false, f3(0.0f), dirv(E0, L1)) * (double)gterm(E0, L1);
+        }
+        if (s == 0) {
+            pl0 = (double)pos_pdf(c, E0);
+        } else {
+            pl0 = (double)vert_pdf(c, L1, s >= 2, s >= 2 ? vnorm(pL2 - L1.p) : f3(0.0f), dirv(L1, E0)) * (double)gterm(L1, E0);
+            if (s == 1) {
+                cm1 = !is_zero(vert_eval<false>(c, L1, false, f3(0.0f), dirv(L1, E0)) * pos_eval_unforced(c, L1));
             } else {
-                A = q.xg(i - 1);
-                const f3 pA2 = i - 2 >= 0 ? q.xp(i - 2) : f3(0.0f);
-                const f3 wiA = i - 2 >= 0 ? vnorm(pA2 - A.p) : f3(0.0f);
-                pl = (double)vert_pdf(c, A, i - 2 >= 0, wiA, dirv(A, B)) * (double)gterm(A, B);
-                ci = cst_at(c, i, n, pA2, A, B, C.p);
+                const BVert L2 = q.xg(s - 2);                   // x_{s-2} = y_{s-2}
+                pe2 = (double)vert_pdf(c, L1, true, dirv(L1, E0), dirv(L1, L2)) * (double)gterm(L1, L2);
+                const f3 pm3 = s >= 3 ? q.xp(s - 3) : f3(0.0f);
+                cm1 = split_nonzero(c, s >= 3, pm3, L2, L1, true, E0.p);
             }
-            r *= pe / pl;
-            if (!is_zero(ci)) inv += r * r;
-            if (i == 0) break;
-            pC2 = C.p;
-            C = B;
-            B = A;
         }
     }
+    double inv = 1.0, r = 1.0;
+    // Towards the light, i = s - 1 .. 0: r *= pE(x_i) / pL(x_i).
+    for (int i = s - 1; i >= 0; --i) {
+        const BFac f = bload_fac(mem, 0, i, owner);
+        const double pe = i == s - 1 ? pe1 : (i == s - 2 ? pe2 : (double)f.rev);
+        if (pe == 0.0) break;
+        const double pl = (double)f.fwd;   // pL(y_i); y_0: its position pdf
+        r *= pe / pl;
+        if (i == s - 1 ? cm1 : f.split) inv += r * r;
+    }
     r = 1.0;
-    if (s + 1 < n) {
-        // Towards the eye, i = s + 1 .. n - 1: A = x_{i-2}, B = x_{i-1}, pA2 = x_{i-3}.p.
-        BVert B = q.xg(s), A = B;   // A is read only once i >= 2
-        f3 pA2(0.0f);
-        if (s >= 1) A = q.xg(s - 1);
-        if (s >= 2) pA2 = q.xp(s - 2);
-        for (int i = s + 1; i < n; ++i) {
-            const int k = i - 1;   // pdf_light_side(k)
-            double pl;
-            if (k == 0) {
-                pl = (double)pos_pdf(c, B);
-            } else {
-                const f3 wiA = k - 2 >= 0 ? vnorm(pA2 - A.p) : f3(0.0f);
-                pl = (double)vert_pdf(c, A, k - 2 >= 0, wiA, dirv(A, B)) * (double)gterm(A, B);
-            }
-            if (pl == 0.0) break;
-            const BVert C = q.xg(i);
-            const f3 pC1 = i + 1 <= n - 1 ? q.xp(i + 1) : f3(0.0f);
-            // pdf_eye_side(k): k <= n - 2.
-            const f3 wiC = i + 1 <= n - 1 ? vnorm(pC1 - C.p) : f3(0.0f);
-            const double pe = (double)vert_pdf(c, C, i + 1 <= n - 1, wiC, dirv(C, B)) * (double)gterm(C, B);
-            r *= pl / pe;
-            if (!is_zero(cst_at(c, i, n, A.p, B, C, pC1))) inv += r * r;
-            pA2 = A.p;
-            A = B;
-            B = C;
-        }
+    // Towards the eye, i = s + 1 .. n - 1: r *= pL(x_{i-1}) / pE(x_{i-1}).
+    for (int i = s + 1; i < n; ++i) {
+        const BFac f = bload_fac(mem, 1, n - i, owner);         // x_{i-1} = z_{n-i}
+        const double pl = i == s + 1 ? pl0 : (i == s + 2 ? pl1 : (double)f.rev);
+        if (pl == 0.0) break;
+        const double pe = (double)f.fwd;
+        r *= pl / pe;
+        if (i == s + 1 ? cp1 : bload_fac(mem, 1, n - 1 - i, owner).split) inv += r * r;
     }
     const double w = 1.0 / inv;
     return (w == w && inv < 1e300) ? w : 0.0;
@@ -331,7 +418,7 @@ __device__ __forceinline__ double mis_weight_window(const BdptCtx& c, const Conn
 
 // Power-heuristic weight of strategy s: 1 / sum_i (p_i / p_s)^2 over the
 // strategies with c_i != 0.  Returns 0 when p_s itself vanishes.  (The
-// definition; the kernel evaluates it through mis_weight_window.)
+// definition; the kernel evaluates it through mis_weight_fac.)
 __device__ __forceinline__ double mis_weight(const BdptCtx& c, const Conn& q) {
     const int s = q.s, n = q.n;
     double inv = 1.0, r = 1.0;
@@ -411,7 +498,7 @@ __device__ __forceinline__ void eval_strategy(const RenderArgs& a, const BdptCtx
     const f3 aE = bload(mem, 1, t - 1, owner).alpha;
     const f3 Cu = aL * cs * aE;
     if (is_zero(Cu)) return;
-    const double w = mis_weight_window(c, q);
+    const double w = mis_weight_fac(c, mem, owner, s, n);
     if (w == 0.0) return;
     float sel = 1.0f;   // SelectionProb (bdpt.hpp:193-208)
     for (int i = 1; i < s - 1; ++i) sel *= rr_at(a, i);
@@ -545,6 +632,8 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
                 }
                 nsub[side] = nv;
             }
+            subpath_factors(c, mem, 0, nsub[0], gl);
+            subpath_factors(c, mem, 1, nsub[1], gl);
         }
         // Every lane reads the other lanes' vertex records below.
         __threadfence_block();

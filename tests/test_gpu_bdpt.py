@@ -79,3 +79,32 @@ def test_bdpt_unbounded_length_with_roulette_and_engine_choice():
         out.append(ngi.render(s, p)[0])
     np.testing.assert_allclose(out[1], out[0], rtol=1e-9, atol=1e-12 * np.abs(out[0]).max())
     _crn_close(out[0], oracle.render(s, _p(16, 16, 8, -1, seed=8)), 5e-4)
+
+
+@pytest.mark.parametrize("spec", [ngi.SPEC_REFRACTION, ngi.SPEC_FRESNEL])
+def test_bdpt_through_glass(spec):
+    """Camera and floor inside a glass box, the light outside: every strategy
+    that connects across the glass is blocked, so the image comes from the
+    strategies whose subpaths cross the refracting interface (their c = 0
+    neighbours drop out of the MIS sum)."""
+    s = scenes.glass_room(spec)
+    p = _p(24, 24, 32, 8, seed=5)
+    g, _ = ngi.render(s, p)
+    o = oracle.render(s, p)
+    assert (o.sum(axis=2) > 0).mean() > 0.2
+    _crn_close(g, o, 5e-4, frac=0.95, mean_rtol=5e-3)
+
+
+@pytest.mark.parametrize("kind,detail", [(ngi.GEN_NAVE, 0), (ngi.GEN_SPHERE_FIELD, 3)])
+def test_bdpt_nave_and_glossy_sphere_field(kind, detail):
+    """Long paths (up to 12 vertices, roulette from 3) over thousands of
+    triangles, diffuse (nave, 4 lights) and Beckmann-glossy patches (sphere
+    field, 17 lights).  A subpath whose FP32 ray hits other geometry than
+    the FP64 one changes every strategy of its sample, so on the nave's long
+    paths 95% of the pixels is the bar (96.5% agree here)."""
+    s = ngi.Scene.generate(kind, 1, 1.0, detail)
+    p = _p(24, 24, 8, 12, rr_depth=3, seed=7)
+    g, st = ngi.render(s, p)
+    o = oracle.render(s, p)
+    assert np.abs(o).max() > 0 and st["shadow_rays"] > 0
+    _crn_close(g, o, 5e-4, frac=0.95, mean_rtol=5e-3)
