@@ -90,3 +90,26 @@ def test_ltdirect_counters_and_engine_choice():
     for g, st in out[1:]:
         np.testing.assert_allclose(g, out[0][0], rtol=1e-9, atol=1e-12 * np.abs(out[0][0]).max())
         assert st["node_visits"] > st["extension_rays"] > 0
+
+
+@pytest.mark.parametrize("spec", [ngi.SPEC_REFRACTION, ngi.SPEC_FRESNEL])
+def test_ltdirect_through_glass(spec):
+    """Light paths enter the glass room through its top and reach the floor
+    (light-to-eye transport: no eta^2 scaling at the interface); every sensor
+    connection from inside the glass reaches the camera inside it."""
+    s = scenes.glass_room(spec)
+    p = _p("ltdirect", 24, 24, 300000, 8, seed=3, rr_depth=3)
+    g, _ = ngi.render(s, p)
+    o = oracle.render(s, p)
+    assert (o.sum(axis=2) > 0).mean() > 0.2
+    _crn_close(g, o, 5e-4, frac=0.95, mean_rtol=5e-3)
+
+
+@pytest.mark.parametrize("kind,detail", [(ngi.GEN_NAVE, 0), (ngi.GEN_SPHERE_FIELD, 3)])
+def test_ltdirect_nave_and_glossy_sphere_field(kind, detail):
+    s = ngi.Scene.generate(kind, 1, 1.0, detail)
+    p = _p("ltdirect", 24, 24, 300000, 10, seed=5, rr_depth=3)
+    g, st = ngi.render(s, p)
+    o = oracle.render(s, p)
+    assert np.abs(o).max() > 0 and st["shadow_rays"] > 0
+    _crn_close(g, o, 5e-4, frac=0.95, mean_rtol=5e-3)
