@@ -80,7 +80,9 @@ __device__ __forceinline__ void grad_add(double* lds, double* glob, int nlds, in
     else unsafeAtomicAdd(glob + 3 * prim + c, (double)v);
 }
 
-template <bool DIRECT, int WIDTH>
+// ZEROS: some parameter channel is zero (the zero bookkeeping is compiled in
+// only then; without it every mask is 0 and z stays 0).
+template <bool DIRECT, int WIDTH, bool ZEROS>
 __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(RenderArgs a, AdjArgs g, unsigned long long* next) {
     extern __shared__ double adj_smem[];
     const int nlds = min(g.nprims, NGI_ADJ_LDS_PRIMS);
@@ -113,7 +115,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
         const float* gp = g.dLdI + 3 * (size_t)pix;
         const float gC[3] = {gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale};
         const DevPrim<float>& E = prims[emitter_prim];
-        const uint32_t zm = g.zmask[emitter_prim];
+        const uint32_t zm = ZEROS ? g.zmask[emitter_prim] : 0u;
         float sa[3] = {SA.x, SA.y, SA.z}, sb[3] = {SB.x, SB.y, SB.z};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -136,7 +138,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
             const uint32_t w = __float_as_uint(e.w);
             const int m = (int)(w & NGI_ADJ_PRIM_MASK);
             const DevPrim<float>& P = prims[m];
-            const uint32_t zm = g.zmask[m];
+            const uint32_t zm = ZEROS ? g.zmask[m] : 0u;
             const float pa[3] = {e.x, e.y, e.z};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
         bool alive = nv < max_nv;
         if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && nrec < g.max_rec) {
             // Vertex reflectance enters here (NEE fsE, then the BSDF weight).
-            const uint32_t zm = g.zmask[prim];
+            const uint32_t zm = ZEROS ? g.zmask[prim] : 0u;
             uint32_t w = (uint32_t)prim;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -256,19 +258,19 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
     }
 }
 
-template <bool DIRECT, int WIDTH>
+template <bool DIRECT, int WIDTH, bool ZEROS>
 static int launch_adjoint(const RenderArgs& a, const AdjArgs& g, unsigned long long* next, unsigned grid, size_t lds, hipStream_t st) {
-    adjoint_kernel<DIRECT, WIDTH><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
+    adjoint_kernel<DIRECT, WIDTH, ZEROS><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
-template <bool DIRECT, int WIDTH>
+template <bool DIRECT, int WIDTH, bool ZEROS>
 static int adjoint_grid(size_t lds, unsigned& grid) {
     int dev = 0, cus = 0, per_cu = 0;
     NGI_HIP_CHECK(hipGetDevice(&dev));
     NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<DIRECT, WIDTH>, NGI_ADJ_BLOCK, lds));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<DIRECT, WIDTH, ZEROS>, NGI_ADJ_BLOCK, lds));
     grid = (unsigned)std::max(1, cus * std::max(1, per_cu));
     return NGI_OK;
 }
@@ -299,21 +301,14 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     const int nlds = std::min(nprims, NGI_ADJ_LDS_PRIMS);
     const size_t lds = (size_t)6 * nlds * 8 + (width == 8 ? (size_t)scene->dev->view.stack_entries * NGI_ADJ_BLOCK * 4 : 0);
     unsigned grid = 1;
-    st = direct ? (width == 8 ? adjoint_grid<true, 8>(lds, grid) : adjoint_grid<true, 2>(lds, grid))
-                : (width == 8 ? adjoint_grid<false, 8>(lds, grid) : adjoint_grid<false, 2>(lds, grid));
     std::vector<void*> mem;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(str);
         for (void* m : mem) (void)hipFree(m);
         S.destroy();
     };
-    if (st) {
-        cleanup();
-        return st;
-    }
     const int max_nv = (p.max_num_vertices < 0 || p.max_num_vertices > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : p.max_num_vertices;
     AdjArgs g;
-    g.lanes = grid * NGI_ADJ_BLOCK;
     g.max_rec = max_nv;
     g.nprims = nprims;
     g.scale = total ? (float)((double)npix / (double)total) : 0.0f;
@@ -346,6 +341,16 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     for (int l = 0; l < view.num_lights; ++l)
         for (int c = 0; c < 3; ++c)
             if (zm[rl[l].prim] & (NGI_ADJ_LEZERO << c)) rl[l].Le[c] = 1.0f;
+    const bool zeros = std::any_of(zm.begin(), zm.end(), [](uint8_t v) { return v != 0; });
+    st = direct ? (width == 8 ? (zeros ? adjoint_grid<true, 8, true>(lds, grid) : adjoint_grid<true, 8, false>(lds, grid))
+                              : (zeros ? adjoint_grid<true, 2, true>(lds, grid) : adjoint_grid<true, 2, false>(lds, grid)))
+                : (width == 8 ? (zeros ? adjoint_grid<false, 8, true>(lds, grid) : adjoint_grid<false, 8, false>(lds, grid))
+                              : (zeros ? adjoint_grid<false, 2, true>(lds, grid) : adjoint_grid<false, 2, false>(lds, grid)));
+    if (st) {
+        cleanup();
+        return st;
+    }
+    g.lanes = grid * NGI_ADJ_BLOCK;
     void *d_dl = nullptr, *d_rec = nullptr, *d_gR = nullptr, *d_gL = nullptr, *d_next = nullptr;
     void *d_rp = nullptr, *d_rl = nullptr, *d_zm = nullptr;
 #define NGI_ADJ_ALLOC(ptr, bytes)                                   \
@@ -389,8 +394,12 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     a.sample_base = 0;
     if (!st && total) {
         unsigned long long* nx = reinterpret_cast<unsigned long long*>(d_next);
-        st = direct ? (width == 8 ? launch_adjoint<true, 8>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2>(a, g, nx, grid, lds, str))
-                    : (width == 8 ? launch_adjoint<false, 8>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2>(a, g, nx, grid, lds, str));
+        if (zeros)
+            st = direct ? (width == 8 ? launch_adjoint<true, 8, true>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2, true>(a, g, nx, grid, lds, str))
+                        : (width == 8 ? launch_adjoint<false, 8, true>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2, true>(a, g, nx, grid, lds, str));
+        else
+            st = direct ? (width == 8 ? launch_adjoint<true, 8, false>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2, false>(a, g, nx, grid, lds, str))
+                        : (width == 8 ? launch_adjoint<false, 8, false>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2, false>(a, g, nx, grid, lds, str));
     }
     if (!st && hipMemcpyAsync(gR_out, d_gR, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");
     if (!st && hipMemcpyAsync(gLe_out, d_gL, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");

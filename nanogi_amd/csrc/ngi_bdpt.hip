@@ -41,14 +41,18 @@ namespace ngi {
 #define NGI_BD_BLOCK 256
 // Minimum waves per SIMD for the bdpt kernel's register allocation: 4
 // (128 VGPRs, a little spilled) measured fastest against 3 (160 VGPRs) and
-// 5: +9% on the Cornell box, +16% on the nave.
+// 5: +9% on the Cornell box, +16% on the nave; with the stored MIS factors
+// 4 / 3 / 5 waves give 201 / 189 / 174 M samples/s (Cornell) and 58.5 /
+// 51.0 / 57.2 M (nave).
 #ifndef NGI_BD_WAVES
 #define NGI_BD_WAVES 4
 #endif
 // Strategies are pooled per path length when that takes at most this many
-// quarters of the rounds of a single pool (5: a quarter more).
+// quarters of the rounds of a single pool.  4 (no more rounds than one pool)
+// measured best with the stored MIS factors: Cornell box 203 / 201 / 200 M
+// samples/s and nave 61.5 / 58.5 / 58.4 M for 4 / 5 / 6 quarters.
 #ifndef NGI_BD_POOL_QUARTERS
-#define NGI_BD_POOL_QUARTERS 5u
+#define NGI_BD_POOL_QUARTERS 4u
 #endif
 // Vertices per subpath; the reference has no cap with -m -1 (with Russian
 // roulette at 0.5 a longer subpath has probability 2^-62).
@@ -646,7 +650,7 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
         // each sample has.  Pooling per path length n gives rounds whose MIS
         // walks all take n steps (Cornell box 131 -> 158 M samples/s); with
         // Russian roulette the per-length pools are small, so a wave pools by
-        // length only when that costs at most a quarter more rounds.
+        // length only when that costs no more rounds than one pool.
         const int nmax = (a.max_nv >= 0 && a.max_nv < nsub[0] + nsub[1]) ? a.max_nv : nsub[0] + nsub[1];
         int wmax = have ? nmax : 0;   // longest path length of the wave
         for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
