@@ -149,10 +149,11 @@ enum { NGI_BVH_2 = 2, NGI_BVH_8 = 8 };
 
 typedef struct ngi_bvh_opts {
     int width;             /* NGI_BVH_2 or NGI_BVH_8 (0 = default 8) */
-    int max_leaf_size;     /* 0 = default */
+    int max_leaf_size;     /* triangles per leaf, at most 16; 0 = default (4) */
     int sah_bins;          /* 0 = default (32) */
     int num_threads;       /* 0 = hardware concurrency */
-    double traversal_cost; /* SAH cost of a node visit relative to one triangle test (0 = default 1.2) */
+    double traversal_cost; /* SAH cost of a node visit relative to one triangle test (0 = default: 1.2 for
+                            * the binary BVH's splits and leaves, 1.0 for the wide BVH's collapse) */
 } ngi_bvh_opts;
 
 typedef struct ngi_bvh_info {

@@ -6,7 +6,8 @@
 // 2007) over triangle centroids, subtrees built in parallel; node order is
 // depth-first pre-order and independent of thread scheduling, so the build
 // is deterministic.  The binary tree is then either emitted as 64-byte
-// binary records or collapsed into 8-wide 128-byte records.
+// binary records or, built down to single triangles, collapsed into 8-wide
+// 128-byte records by an SAH-optimal dynamic programme.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -52,6 +53,16 @@ struct Ctx {
     std::atomic<int> spare_threads{0};
     double ct = 1.2, ci = 1.0;     // relative traversal / intersection cost
 };
+
+// SAH cost of a wide-node visit in units of one triangle test, for the
+// collapse into 8-wide nodes (default of ngi_bvh_opts.traversal_cost there).
+// Measured on the 1.8M-triangle sphere field / the nave (ptdirect, samples/s
+// on one MI355X): 1,454 / 593 M at 1.0, 1,441 / 599 M at 1.5, 1,440 / 596 M
+// at 2.0, 1,420 / 593 M at 2.5, 1,396 M (C3) at 3.0; the greedy collapse
+// (open the child of largest area until eight) 1,451 / 582 M.
+#ifndef NGI_BVH8_NODE_COST
+#define NGI_BVH8_NODE_COST 1.0
+#endif
 
 const int kMedianDepth = 36;        // beyond this depth: object-median splits (bounds depth by 36 + log2(n))
 
@@ -221,6 +232,83 @@ void emit_bvh2(const std::vector<Node>& nodes, Bvh& out) {
 struct WideTmp {
     int32_t bin;                 // binary node this wide node represents
     std::vector<int32_t> kids;   // binary node indices of the children
+    std::vector<uint8_t> leaf;   // per kid: a leaf over the kid's whole subtree
+};
+
+// Optimal collapse of the binary tree into 8-wide nodes (after Ylitie,
+// Karras, Laine, "Efficient Incoherent Ray Traversal on GPUs Through
+// Compressed Wide BVHs", HPG 2017, section 3.1).  For every binary node n and
+// i = 1..8 child slots, C(n, i) is the least SAH cost of n's subtree spread
+// over at most i slots of one wide node:
+//   leaf      A(n) * tris(n)                     (tris(n) <= max_leaf)
+//   wide node A(n) * c_node + D(n, 8)           (one slot holding a new node)
+//   opened    D(n, i) = min_k C(l, k) + C(r, i - k), i >= 2
+// and C(n, i) <= C(n, i - 1).  c_node is the cost of a wide-node visit in
+// units of one triangle test.
+struct Collapse {
+    struct How {
+        uint8_t kind;   // 0 leaf, 1 wide node, 2 opened
+        uint8_t kl, kr; // opened: slots given to the left and right child
+    };
+    std::vector<float> cost;      // C(n, i) at [8 n + i - 1]
+    std::vector<How> how;
+    std::vector<uint8_t> split8;  // k of the best D(n, 8)
+    std::vector<uint32_t> first, tris;
+
+    void run(const std::vector<Node>& nodes, int max_leaf, double c_node) {
+        const size_t n = nodes.size();
+        cost.assign(8 * n, 0.0f);
+        how.assign(8 * n, How{0, 0, 0});
+        split8.assign(n, 1);
+        first.assign(n, 0);
+        tris.assign(n, 0);
+        const double inv_root = 1.0 / std::max(nodes[0].box.area(), 1e-30);
+        for (size_t q = n; q-- > 0;) {   // pre-order: children after their parent
+            const Node& nd = nodes[q];
+            const double A = nd.box.area() * inv_root;
+            if (nd.count) {
+                first[q] = nd.start, tris[q] = nd.count;
+                const float c = (float)(A * nd.count);
+                for (int i = 0; i < 8; ++i) cost[8 * q + i] = c, how[8 * q + i] = How{0, 0, 0};
+                continue;
+            }
+            const size_t l = (size_t)nd.left, r = (size_t)nd.right;
+            first[q] = first[l], tris[q] = tris[l] + tris[r];
+            float D[9];
+            uint8_t Dk[9];
+            for (int j = 2; j <= 8; ++j) {
+                D[j] = INFINITY, Dk[j] = 1;
+                for (int k = 1; k < j; ++k) {
+                    const float c = cost[8 * l + k - 1] + cost[8 * r + (j - k) - 1];
+                    if (c < D[j]) D[j] = c, Dk[j] = (uint8_t)k;
+                }
+            }
+            split8[q] = Dk[8];
+            const float leafc = tris[q] <= (uint32_t)max_leaf ? (float)(A * tris[q]) : INFINITY;
+            const float widec = (float)(A * c_node) + D[8];
+            float best = leafc;
+            How h{0, 0, 0};
+            if (widec < best) best = widec, h = How{1, 0, 0};
+            cost[8 * q] = best, how[8 * q] = h;
+            for (int i = 2; i <= 8; ++i) {
+                float c = cost[8 * q + i - 2];
+                How hi = how[8 * q + i - 2];
+                if (D[i] < c) c = D[i], hi = How{2, Dk[i], (uint8_t)(i - Dk[i])};
+                cost[8 * q + i - 1] = c, how[8 * q + i - 1] = hi;
+            }
+        }
+    }
+    // Slot contents for binary node q given i slots: (node, as-leaf) pairs.
+    void gather(int32_t q, int i, const std::vector<Node>& nodes, WideTmp& w) const {
+        const How h = how[8 * (size_t)q + i - 1];
+        if (h.kind == 2) {
+            gather(nodes[q].left, h.kl, nodes, w);
+            gather(nodes[q].right, h.kr, nodes, w);
+            return;
+        }
+        w.kids.push_back(q);
+        w.leaf.push_back(h.kind == 0 ? 1 : 0);
+    }
 };
 
 // Quantise child boxes conservatively on a per-axis power-of-two grid.
@@ -262,29 +350,20 @@ void quantise(const Box& parent, const std::vector<Box>& kids, uint8_t valid, Bv
     }
 }
 
-void emit_bvh8(const std::vector<Node>& nodes, Bvh& out) {
+void emit_bvh8(const std::vector<Node>& nodes, int max_leaf, double c_node, Bvh& out) {
+    Collapse col;
+    col.run(nodes, max_leaf, c_node);
     std::vector<WideTmp> wide;
     auto make = [&](int32_t bin) {
         WideTmp w;
         w.bin = bin;
-        if (nodes[bin].count) {
-            w.kids.push_back(bin);   // root leaf: single child
+        if (nodes[bin].count || col.how[8 * (size_t)bin].kind == 0) {
+            // Root leaf: a single leaf child over the whole tree.
+            w.kids.push_back(bin);
+            w.leaf.push_back(1);
         } else {
-            w.kids = {nodes[bin].left, nodes[bin].right};
-            while (w.kids.size() < 8) {
-                int best = -1;
-                double ba = -1;
-                for (size_t k = 0; k < w.kids.size(); ++k) {
-                    const Node& nd = nodes[w.kids[k]];
-                    if (nd.count) continue;
-                    const double a = nd.box.area();
-                    if (a > ba) ba = a, best = (int)k;
-                }
-                if (best < 0) break;
-                const int32_t open = w.kids[best];
-                w.kids[best] = nodes[open].left;
-                w.kids.insert(w.kids.begin() + best + 1, nodes[open].right);
-            }
+            col.gather(nodes[bin].left, col.split8[bin], nodes, w);
+            col.gather(nodes[bin].right, 8 - col.split8[bin], nodes, w);
         }
         return w;
     };
@@ -299,7 +378,7 @@ void emit_bvh8(const std::vector<Node>& nodes, Bvh& out) {
         wide.push_back(make(bin));
         const WideTmp& w = wide.back();
         for (int k = (int)w.kids.size() - 1; k >= 0; --k)
-            if (nodes[w.kids[k]].count == 0 && w.kids[k] != bin) st.push_back(w.kids[k]);
+            if (!w.leaf[k]) st.push_back(w.kids[k]);
     }
     out.nodes.assign(wide.size() * sizeof(Bvh8Node), 0);
     Bvh8Node* R = reinterpret_cast<Bvh8Node*>(out.nodes.data());
@@ -349,8 +428,8 @@ void emit_bvh8(const std::vector<Node>& nodes, Bvh& out) {
                 r.child[sl] = NGI_BVH_EMPTY;
                 continue;
             }
-            const Node& nd = nodes[w.kids[kid_in[sl]]];
-            r.child[sl] = nd.count ? ngi_leaf_ref(nd.start, nd.count) : wide_index_of_bin[w.kids[kid_in[sl]]];
+            const int32_t kid = w.kids[kid_in[sl]];
+            r.child[sl] = w.leaf[kid_in[sl]] ? ngi_leaf_ref(col.first[kid], col.tris[kid]) : wide_index_of_bin[kid];
         }
     }
     // Depth of the wide tree bounds the traversal stack (one entry per level).
@@ -367,6 +446,11 @@ void emit_bvh8(const std::vector<Node>& nodes, Bvh& out) {
     out.max_depth = wmax;
     out.num_nodes = (uint32_t)wide.size();
     out.node_bytes = sizeof(Bvh8Node);
+    uint32_t leaves = 0;
+    for (const WideTmp& w : wide)
+        for (uint8_t f : w.leaf) leaves += f;
+    out.num_leaves = leaves;
+    out.sah_cost = nodes[0].count ? col.cost[0] : (double)col.cost[7] + c_node;   // root: a wide node of cost c_node
 }
 
 }  // namespace
@@ -384,7 +468,10 @@ int bvh_build(const HostScene& s, const ngi_bvh_opts& o, Bvh& out) {
     }
     if (nt >= (1u << 27)) return fail(NGI_ERR_INVALID_ARG, "too many triangles (%u)", nt);
     Ctx c;
-    c.max_leaf = o.max_leaf_size > 0 ? std::min(o.max_leaf_size, 16) : (width == NGI_BVH_8 ? 4 : 4);
+    // Wide BVHs: the binary tree goes down to single triangles and the
+    // optimal collapse chooses the leaves (at most max_leaf triangles each).
+    const int max_leaf = o.max_leaf_size > 0 ? std::min(o.max_leaf_size, 16) : 4;
+    c.max_leaf = width == NGI_BVH_8 ? 1 : max_leaf;
     c.bins = o.sah_bins > 0 ? std::min(std::max(o.sah_bins, 4), 256) : 32;
     if (o.traversal_cost < 0 || o.traversal_cost > 1e3) return fail(NGI_ERR_INVALID_ARG, "traversal_cost out of range");
     if (o.traversal_cost > 0) c.ct = o.traversal_cost;
@@ -428,7 +515,7 @@ int bvh_build(const HostScene& s, const ngi_bvh_opts& o, Bvh& out) {
         emit_bvh2(nodes, out);
         if (out.max_depth >= 64) return fail(NGI_ERR_INVALID_SCENE, "binary BVH depth %u exceeds the traversal stack", out.max_depth);
     } else {
-        emit_bvh8(nodes, out);
+        emit_bvh8(nodes, max_leaf, o.traversal_cost > 0 ? o.traversal_cost : NGI_BVH8_NODE_COST, out);
         if (out.max_depth >= 32) return fail(NGI_ERR_INVALID_SCENE, "wide BVH depth %u exceeds the traversal stack", out.max_depth);
         if (out.num_nodes >= (1u << 24)) return fail(NGI_ERR_INVALID_SCENE, "too many wide BVH nodes (%u)", out.num_nodes);
     }
