@@ -321,15 +321,17 @@ __device__ __forceinline__ void path_start(const RenderArgs& a, uint64_t id, con
     ps.nv = 1;
 }
 
-// One iteration of the vertex loop (src/nanogi.cpp:483-606 / 645-801):
-// NEE (ptdirect), direction sample, extension ray, emission (pt), Russian
-// roulette.  Returns false when the path ends.  With QUEUE the NEE shadow ray
-// is appended to the wavefront's shadow queue at slot qtail + (rank among the
-// lanes appending) and *pushed is set, instead of being traced here.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
-__device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
-                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
-                                            uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
+// One iteration of the vertex loop (src/nanogi.cpp:483-606 / 645-801) in two
+// halves around the extension ray.  vertex_begin: NEE (ptdirect) and the
+// direction sample; returns false when the path ends here, else the
+// direction wo and the roulette number urr of the vertex.  With QUEUE the NEE
+// shadow ray is appended to the wavefront's shadow queue at slot qtail +
+// (rank among the lanes appending) and *pushed is set, instead of being
+// traced here.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE>
+__device__ __forceinline__ bool vertex_begin(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
+                                             uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q, uint32_t qwave,
+                                             uint32_t qtail, bool* pushed, V3<R>& wo, R& urr) {
     const DevScene& sc = a.sc;
     if (ps.nv >= max_nv) return false;
     if (DIRECT) {
@@ -352,8 +354,8 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
         }
     }
     const U4 rx = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_EXT, a.seed, a.stream_id);
-    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
-    V3<R> wo;
+    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z);
+    urr = unif<R>(rx.w);
     if (ps.type == NGI_PRIM_E) {
         wo = camera_sample<R>(a, cam, u0, u1, ps.pixel);
     } else {
@@ -361,16 +363,26 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
         if (!bsdf_sample<R>(prims[ps.prim], ps.type, ps.s, ps.wi, u0, u1, uc, wo, wgt)) return false;
         ps.T = ps.T * wgt;
     }
+    return true;
+}
+
+// Extension ray of a vertex: from the vertex along wo, hit interval (EpsF, inf).
+template <typename R>
+__device__ __forceinline__ TraceRay extension_ray(const PathState<R>& ps, V3<R> wo) {
     TraceRay er;
     er.ox = (float)ps.s.p.x, er.oy = (float)ps.s.p.y, er.oz = (float)ps.s.p.z;
     er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
     er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
-    TraceHit h;
-    TraceCounters tc{0, 0};
-    const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, stk, stride);
-    cnt[CNT_EXT_RAYS]++;
-    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
-    if (!hit) return false;
+    return er;
+}
+
+// Second half: the hit of the extension ray (h.leaf = -1 for a miss),
+// emission (pt), Russian roulette.  Returns false when the path ends.
+template <typename R, typename FilmT, bool DIRECT>
+__device__ __forceinline__ bool vertex_end(const RenderArgs& a, const DevPrim<R>* prims, PathState<R>& ps, V3<R> wo, R urr,
+                                           const TraceHit& h) {
+    const DevScene& sc = a.sc;
+    if (h.leaf < 0) return false;
     int hp;
     const Surf<R> ns = hit_surface<R>(sc, ps.s.p, wo, h, hp);
     const DevPrim<R>& hm = prims[hp];
@@ -392,6 +404,24 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
     ps.wi = -wo;
     ps.nv++;
     return true;
+}
+
+// The whole vertex with the extension ray traced in place.  Returns false
+// when the path ends.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
+__device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, PathState<R>& ps,
+                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
+                                            uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
+    V3<R> wo;
+    R urr;
+    if (!vertex_begin<R, FilmT, DIRECT, WIDTH, COUNT, QUEUE>(a, cam, prims, ps, stk, stride, max_nv, cnt, q, qwave, qtail, pushed, wo, urr))
+        return false;
+    TraceHit h;
+    TraceCounters tc{0, 0};
+    trace<WIDTH, false, COUNT>(a.sc, extension_ray(ps, wo), h, tc, stk, stride);
+    cnt[CNT_EXT_RAYS]++;
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    return vertex_end<R, FilmT, DIRECT>(a, prims, ps, wo, urr, h);
 }
 
 __device__ __forceinline__ int clamp_vertices(int m) { return (m < 0 || m > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : m; }

@@ -107,6 +107,140 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
     flush_counters(a.counters, cnt, COUNT);
 }
 
+// The persistent kernel for FP32 shading over the wide BVH, with extension
+// traversals that outlive a wavefront's shading step.  A wavefront stops
+// traversing as soon as at most NGI_PK_KEEP of its lanes are still inside the
+// tree (instead of waiting for its slowest ray); the lanes whose rays have
+// finished complete their vertices, take new paths and start their next rays,
+// while the others keep their traversal state and carry on in the next
+// traversal step alongside the new rays.  Between traversal steps a lane's
+// traversal state, direction and roulette number are parked in LDS (the
+// shading code needs the registers), next to its traversal stack; shadow rays
+// use a second LDS stack per lane.  Every path draws the same random numbers
+// and makes the same node visits and triangle tests as in persistent_kernel;
+// the film receives its additions in another order.
+//
+// Waiting for every lane kept 31% of the lanes busy per traversal iteration
+// on the 1.8M-triangle sphere field; stopping at 16 lanes left, 68%.
+// Measured (M samples/s, sphere field / nave, ptdirect, 64 spp per launch):
+// waiting for all 1,455 / 597, keep 4: 1,612 / 655, 8: 1,639 / 674,
+// 12: 1,638 / 682, 16: 1,616 / 684, 24: 1,543 / 662 (more keep, more shading
+// steps with fewer lanes).
+#ifndef NGI_PK_KEEP
+#define NGI_PK_KEEP 12
+#endif
+enum { PK_CUR = 0, PK_MASK, PK_SP, PK_T, PK_U, PK_V, PK_LEAF, PK_WX, PK_WY, PK_WZ, PK_URR, PK_WORDS };
+#define NGI_CU_LDS_BYTES (160u * 1024u)
+
+__device__ __forceinline__ void park_trav(uint32_t* pk, const Trav8& t) {
+    pk[PK_CUR * NGI_MK_BLOCK] = t.cur;
+    pk[PK_MASK * NGI_MK_BLOCK] = t.mask;
+    pk[PK_SP * NGI_MK_BLOCK] = (uint32_t)t.sp;
+    pk[PK_T * NGI_MK_BLOCK] = __float_as_uint(t.h.t);
+    pk[PK_U * NGI_MK_BLOCK] = __float_as_uint(t.h.u);
+    pk[PK_V * NGI_MK_BLOCK] = __float_as_uint(t.h.v);
+    pk[PK_LEAF * NGI_MK_BLOCK] = (uint32_t)t.h.leaf;
+}
+__device__ __forceinline__ TraceHit parked_hit(const uint32_t* pk) {
+    TraceHit h;
+    h.t = __uint_as_float(pk[PK_T * NGI_MK_BLOCK]);
+    h.u = __uint_as_float(pk[PK_U * NGI_MK_BLOCK]);
+    h.v = __uint_as_float(pk[PK_V * NGI_MK_BLOCK]);
+    h.leaf = (int32_t)pk[PK_LEAF * NGI_MK_BLOCK];
+    return h;
+}
+__device__ __forceinline__ Trav8 parked_trav(const uint32_t* pk) {
+    Trav8 t;
+    t.cur = pk[PK_CUR * NGI_MK_BLOCK];
+    t.mask = pk[PK_MASK * NGI_MK_BLOCK];
+    t.sp = (int)pk[PK_SP * NGI_MK_BLOCK];
+    t.h = parked_hit(pk);
+    return t;
+}
+__device__ __forceinline__ V3<float> parked_dir(const uint32_t* pk) {
+    return V3<float>(__uint_as_float(pk[PK_WX * NGI_MK_BLOCK]), __uint_as_float(pk[PK_WY * NGI_MK_BLOCK]),
+                     __uint_as_float(pk[PK_WZ * NGI_MK_BLOCK]));
+}
+
+template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
+__global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES) resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    using R = float;
+    constexpr bool QUEUE = DIRECT;
+    // LDS: extension stacks, shadow stacks (a.sc.stack_entries words per lane
+    // each), then the parked lane state (PK_WORDS per lane).
+    extern __shared__ uint32_t stk[];
+    uint32_t* ext_stk = stk + threadIdx.x;
+    uint32_t* sh_stk = stk + (size_t)a.sc.stack_entries * NGI_MK_BLOCK + threadIdx.x;
+    uint32_t* pk = stk + (size_t)2 * a.sc.stack_entries * NGI_MK_BLOCK + threadIdx.x;
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    TraceCounters tc{0, 0};
+    const Cam<R> cam = load_cam<R>(a.sc.cam);
+    const DevPrim<R>* prims = prims_of<R>(a.sc);
+    const int max_nv = clamp_vertices(a.max_nv);
+    const uint32_t wave = blockIdx.x * (NGI_MK_BLOCK / 64) + (threadIdx.x >> 6);
+    const bool geometry = a.sc.num_tris != 0;
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform: oldest slot, queued rays (< NGI_SHQ)
+    PathState<R> ps;
+    PathFeed feed;
+    bool active = false;   // the lane holds a path
+    bool busy = false;     // ... whose extension ray is being traced
+    for (;;) {
+        // Vertices whose extension rays have finished.
+        if (active && !busy)
+            active = vertex_end<R, FilmT, DIRECT>(a, prims, ps, parked_dir(pk), __uint_as_float(pk[PK_URR * NGI_MK_BLOCK]), parked_hit(pk));
+        if (!feed.done()) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
+                if (!active && id != ~0ull) {
+                    path_start<R>(a, a.path_begin + id, cam, ps);
+                    cnt[CNT_SAMPLES]++;
+                    active = true;
+                }
+            }
+        }
+        // Next vertex of every path that is not inside the tree.
+        bool pushed = false;
+        if (active && !busy) {
+            V3<R> wo;
+            R urr;
+            if (vertex_begin<R, FilmT, DIRECT, 8, COUNT, QUEUE>(a, cam, prims, ps, sh_stk, NGI_MK_BLOCK, max_nv, cnt, &q, wave, qhead + qcount,
+                                                                 &pushed, wo, urr)) {
+                cnt[CNT_EXT_RAYS]++;
+                Trav8 tv;
+                trav8_begin(FLT_MAX, tv);
+                busy = geometry;
+                park_trav(pk, tv);
+                pk[PK_WX * NGI_MK_BLOCK] = __float_as_uint(wo.x);
+                pk[PK_WY * NGI_MK_BLOCK] = __float_as_uint(wo.y);
+                pk[PK_WZ * NGI_MK_BLOCK] = __float_as_uint(wo.z);
+                pk[PK_URR * NGI_MK_BLOCK] = __float_as_uint(urr);
+            } else {
+                active = false;
+            }
+        }
+        const bool any = __ballot(active) != 0;
+        if (QUEUE) {
+            qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
+            shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
+        }
+        if (!any) {
+            if (feed.done()) break;
+            continue;
+        }
+        if (__ballot(busy)) {
+            // Lanes without a ray load and store back what they parked last.
+            Trav8 tv = parked_trav(pk);
+            const RayPre r = make_pre(extension_ray(ps, parked_dir(pk)));
+            const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_PK_KEEP);
+            trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_MK_BLOCK);
+            park_trav(pk, tv);
+        }
+    }
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    flush_counters(a.counters, cnt, COUNT);
+}
+
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static void launch_mk(const RenderArgs& a, hipStream_t st) {
     const unsigned grid = (unsigned)((a.path_count + NGI_MK_BLOCK - 1) / NGI_MK_BLOCK);
@@ -127,8 +261,17 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
-    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4 : 0;
-    auto kern = a.path_chunk ? persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>;
+    // Resumable traversals need a second LDS stack and the parked state per
+    // lane: used while that leaves the kernel its full occupancy (on the
+    // 1.8M-triangle sphere field the wide BVH is 9 levels deep: 31 KiB per
+    // block, 5 blocks per CU).
+    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + PK_WORDS) * NGI_MK_BLOCK * 4;
+    const unsigned blocks_per_cu = a.path_chunk && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES;
+    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
+    const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4) : 0;
+    auto kern = resume ? (a.path_chunk ? resume_kernel<FilmT, DIRECT, COUNT, true> : resume_kernel<FilmT, DIRECT, COUNT, false>)
+                       : (a.path_chunk ? persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true>
+                                       : persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>);
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_MK_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;

@@ -324,4 +324,64 @@ __device__ bool trace_bvh8(const DevScene& sc, const TraceRay& ray, TraceHit& h,
     return h.leaf >= 0;
 }
 
+// Resumable closest-hit traversal of the wide BVH: the same visiting order
+// and tests as trace_bvh8<false>, with the state kept by the caller so that a
+// wavefront can stop traversing while some of its lanes are still inside the
+// tree and resume them later (their stacks stay in LDS meanwhile).  The hit
+// distance doubles as the interval's upper end: h.t starts at tmax.
+struct Trav8 {
+    uint32_t cur, mask;
+    int sp;
+    TraceHit h;
+};
+
+// A traversal about to visit the root (the root is visited by the first
+// trav8_run, with the other lanes' next nodes).
+#define NGI_TRAV8_FRESH 0x100u
+__device__ __forceinline__ void trav8_begin(float tmax, Trav8& t) {
+    t.h.t = tmax;
+    t.h.u = t.h.v = 0.0f;
+    t.h.leaf = -1;
+    t.sp = 0;
+    t.cur = 0;
+    t.mask = NGI_TRAV8_FRESH;
+}
+
+// Advances the traversals of the lanes with `busy` set, one node visit or
+// leaf per lane and iteration (a lane whose current node has no pending
+// children first pops its stack: every stack entry holds at least one), until
+// at most `keep` lanes of the wavefront are still busy (at least one
+// iteration; keep = 0 finishes them all).  A lane clears `busy` when its
+// traversal ends; t.h then holds its closest hit or t.h.leaf = -1.
+template <bool COUNT>
+__device__ __forceinline__ void trav8_run(const DevScene& sc, const RayPre& r, float tmin, Trav8& t, bool& busy, int keep,
+                                          TraceCounters& cnt, uint32_t* lds_stack, int lane_stride) {
+    if (busy && t.mask == NGI_TRAV8_FRESH) t.mask = visit8<COUNT>(sc, r, 0, tmin, t.h.t, cnt);
+    do {
+        if (busy && t.mask == 0) {
+            if (t.sp == 0) {
+                busy = false;
+            } else {
+                const uint32_t e = lds_stack[(--t.sp) * lane_stride];
+                t.cur = e >> 8;
+                t.mask = e & 0xffu;
+            }
+        }
+        if (busy) {
+            const uint32_t i = __builtin_ctz(t.mask);
+            t.mask &= t.mask - 1u;
+            const uint32_t slot = i ^ r.octant;
+            const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)t.cur + 4)[slot];
+            if (ref >= 0) {
+                if (t.mask) lds_stack[(t.sp++) * lane_stride] = (t.cur << 8) | t.mask;
+                t.cur = (uint32_t)ref;
+                t.mask = visit8<COUNT>(sc, r, t.cur, tmin, t.h.t, cnt);
+            } else {
+                bool done = false;
+                leaf_test<false, COUNT>(sc, r, ref, tmin, t.h.t, t.h, cnt, done);
+            }
+        }
+    } while (__popcll(__ballot(busy)) > (unsigned)keep);
+}
+
 }  // namespace ngi
