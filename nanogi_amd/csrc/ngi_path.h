@@ -81,13 +81,16 @@ __device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, Tra
 // additions changes.  Record k of slot s of wavefront w is
 // rec[(k * waves + w) * NGI_SHQ + s]: (o, tmax), (d, C.r), (C.g, C.b, pixel, -).
 // A vertex step appends at most 64 rays to a queue holding at most 63.
+#ifndef NGI_SHQ
 #define NGI_SHQ 128
+#endif
 struct ShadowQueue {
     float4* rec;
     uint32_t waves;
+    uint32_t cap = NGI_SHQ;   // records per ring, a power of two >= NGI_SHQ
 };
 __device__ __forceinline__ float4* shq_at(const ShadowQueue& q, int k, uint32_t wave, uint32_t slot) {
-    return q.rec + ((size_t)k * q.waves + wave) * NGI_SHQ + (slot & (NGI_SHQ - 1u));
+    return q.rec + ((size_t)k * q.waves + wave) * q.cap + (slot & (q.cap - 1u));
 }
 // Appends this lane's shadow ray at `slot`.
 __device__ __forceinline__ void shq_push(const ShadowQueue& q, uint32_t wave, uint32_t slot, const TraceRay& r, V3<float> C, int pixel) {
@@ -127,6 +130,88 @@ __device__ __forceinline__ void shq_flush(const RenderArgs& a, const ShadowQueue
         head += n;
         count -= n;
     }
+}
+
+// Traces the queued shadow rays of a wavefront (wide BVH) with refills: each
+// lane takes a queued ray, and lanes whose rays have finished take the next
+// queued rays as soon as NGI_SHQ_REFILL of them are idle, so the wavefront
+// waits for its slowest ray once per drain instead of after every 64.  Starts
+// once the ring cannot take another 64 rays (or with any waiting, with `all`)
+// and returns with the queue empty.  Visits, tests and contributions are
+// those of shq_trace.
+#ifndef NGI_SHQ_REFILL
+#define NGI_SHQ_REFILL 16
+#endif
+template <typename FilmT, bool COUNT>
+__device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t& head, uint32_t& count,
+                                          bool all, uint32_t* stk, int stride, uint32_t* cnt) {
+    if (count + 64u < q.cap && !(all && count > 0)) return;
+    const DevScene& sc = a.sc;
+    const uint32_t lane = __lane_id();
+    bool busy = false;
+    RayPre r;
+    float tmax = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int pixel = 0;
+    uint32_t cur = 0, mask = 0;
+    int sp = 0;
+    TraceCounters tc{0, 0};
+    for (;;) {
+        const unsigned long long idle = __ballot(!busy);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (count > 0 && (nidle >= NGI_SHQ_REFILL || nidle == 64u)) {
+            const uint32_t take = nidle < count ? nidle : count;
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!busy && rank < take) {
+                const uint32_t slot = head + rank;
+                const float4 r0 = *shq_at(q, 0, wave, slot), r1 = *shq_at(q, 1, wave, slot), r2 = *shq_at(q, 2, wave, slot);
+                TraceRay sr;
+                sr.ox = r0.x, sr.oy = r0.y, sr.oz = r0.z, sr.tmax = r0.w;
+                sr.dx = r1.x, sr.dy = r1.y, sr.dz = r1.z;
+                sr.tmin = NGI_EPS_F;
+                r = make_pre(sr);
+                tmax = r0.w;
+                cr = r1.w, cg = r2.x, cb = r2.y;
+                pixel = __float_as_int(r2.z);
+                cur = 0;
+                sp = 0;
+                mask = sc.num_tris ? visit8<COUNT>(sc, r, 0, NGI_EPS_F, tmax, tc) : 0u;
+                busy = true;
+                cnt[CNT_SHADOW_RAYS]++;
+            }
+            head += take;
+            count -= take;
+        }
+        if (__ballot(busy) == 0) break;
+        // One node visit or leaf per lane, after at most one pop (Scene::Visible,
+        // rt.hpp:2251-2261; src/nanogi.cpp:700-706).
+        if (busy && mask == 0) {
+            if (sp == 0) {
+                busy = false;   // nothing in the way
+                film_add<FilmT>(a.film, pixel, V3<float>(cr, cg, cb));
+            } else {
+                const uint32_t e = stk[(--sp) * stride];
+                cur = e >> 8;
+                mask = e & 0xffu;
+            }
+        }
+        if (busy) {
+            const uint32_t i = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t slot = i ^ r.octant;
+            const int32_t ref = reinterpret_cast<const int32_t*>(sc.nodes + 8 * (size_t)cur + 4)[slot];
+            if (ref >= 0) {
+                if (mask) stk[(sp++) * stride] = (cur << 8) | mask;
+                cur = (uint32_t)ref;
+                mask = visit8<COUNT>(sc, r, cur, NGI_EPS_F, tmax, tc);
+            } else {
+                TraceHit h;
+                bool occluded = false;
+                leaf_test<true, COUNT>(sc, r, ref, NGI_EPS_F, tmax, h, tc, occluded);
+                if (occluded) busy = false;
+            }
+        }
+    }
+    if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
 }
 
 // Surface at a hit: p = o + d t in the shading precision (rt.hpp:2197),

@@ -129,6 +129,15 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 12
 #endif
+// Shadow rings of the resumable kernel on large scenes: NGI_SHQ_LONG records
+// per wavefront, drained with refills once they cannot take another 64 rays
+// (126 MB of rings at full occupancy).  Measured (M samples/s, sphere field /
+// nave): rounds of 64 from a 128-ring 1,638 / 682; drains of 128-ring 1,621 /
+// 663, 256: 1,721 / 704, 512: 1,737 / 717, 1024: 1,746 / 704.  Small scenes
+// (chunked path feed) keep rounds of 64: the drain cost 4% on the Cornell box.
+#ifndef NGI_SHQ_LONG
+#define NGI_SHQ_LONG 512
+#endif
 enum { PK_CUR = 0, PK_MASK, PK_SP, PK_T, PK_U, PK_V, PK_LEAF, PK_WX, PK_WY, PK_WZ, PK_URR, PK_WORDS };
 #define NGI_CU_LDS_BYTES (160u * 1024u)
 
@@ -221,8 +230,9 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
         }
         const bool any = __ballot(active) != 0;
         if (QUEUE) {
-            qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
-            shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
+            qcount += (uint32_t)__popcll(__ballot(pushed));   // at most q.cap - 65 + 64 (63 + 64 chunked)
+            if (CHUNKED) shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
+            else shq_drain<FilmT, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
         }
         if (!any) {
             if (feed.done()) break;
@@ -279,7 +289,13 @@ static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     q.waves = grid * (NGI_MK_BLOCK / 64);
     q.rec = nullptr;
     if (DIRECT && std::is_same<R, float>::value) {
-        if ((s = mem.reserve((size_t)3 * q.waves * NGI_SHQ * sizeof(float4)))) return s;
+        // Longer rings for the refilling drains; a caller-owned buffer gets
+        // the longest ring it holds.
+        if (resume && !a.path_chunk) {
+            q.cap = NGI_SHQ_LONG;
+            while (mem.fixed && q.cap > NGI_SHQ && (size_t)3 * q.waves * q.cap * sizeof(float4) > *mem.bytes) q.cap >>= 1;
+        }
+        if ((s = mem.reserve((size_t)3 * q.waves * q.cap * sizeof(float4)))) return s;
         q.rec = reinterpret_cast<float4*>(*mem.ptr);
     }
     kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next, q);
