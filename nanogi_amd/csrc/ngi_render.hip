@@ -121,13 +121,15 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // the film receives its additions in another order.
 //
 // Waiting for every lane kept 31% of the lanes busy per traversal iteration
-// on the 1.8M-triangle sphere field; stopping at 16 lanes left, 68%.
-// Measured (M samples/s, sphere field / nave, ptdirect, 64 spp per launch):
-// waiting for all 1,455 / 597, keep 4: 1,612 / 655, 8: 1,639 / 674,
-// 12: 1,638 / 682, 16: 1,616 / 684, 24: 1,543 / 662 (more keep, more shading
-// steps with fewer lanes).
+// on the 1.8M-triangle sphere field; stopping at 8 lanes left, 63%, with 1.13
+// traversal steps per ray.  Measured (M samples/s, sphere field / nave,
+// ptdirect, 64 spp per launch), shadow rays in rounds of 64: waiting for all
+// 1,455 / 597, keep 4: 1,612 / 655, 8: 1,639 / 674, 12: 1,638 / 682, 16:
+// 1,616 / 684, 24: 1,543 / 662 (more keep, more shading steps with fewer
+// lanes); with the drained 512-record shadow rings: keep 8: 1,753 / 709,
+// 12: 1,738 / 717, 16: 1,704 / 715.
 #ifndef NGI_PK_KEEP
-#define NGI_PK_KEEP 12
+#define NGI_PK_KEEP 8
 #endif
 // Shadow rings of the resumable kernel on large scenes: NGI_SHQ_LONG records
 // per wavefront, drained with refills once they cannot take another 64 rays
