@@ -583,7 +583,7 @@ def main():
             if engine == ngi.ENGINE_PERSISTENT:
                 avg_ms = st["ms_shade"] / a.steps          # one launch per step
                 paths_per_launch = a.spp_per_step * npix
-                kernel = "persistent path kernel (path regeneration)"
+                kernel = "persistent path kernel (path regeneration, resumable traversals)"
             else:
                 mk_launches = max(1, math.ceil(samples / max(1, (1 << 22) // npix * npix)))
                 avg_ms = st["ms_shade"] / mk_launches
