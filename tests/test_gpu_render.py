@@ -247,6 +247,24 @@ def test_persistent_engine_matches_megakernel(renderer):
     _crn_close(g, m, 2e-4)
 
 
+def test_deep_wide_bvh_takes_full_length_traversals():
+    """A wide BVH 10 levels deep (5.8M triangles) needs more LDS per lane
+    for resumable traversals (two stacks of 11 entries and the parked state)
+    than 5 wavefronts per SIMD leave, so FP32 renders fall back to the kernel
+    that runs every traversal to its end; it must agree with the megakernel
+    path by path."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=9)
+    s.build(ngi.BVH_8)
+    assert s.bvh_info()["max_depth"] >= 10
+    out = {}
+    for e in (ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT):
+        out[e] = ngi.render(s, _params("ptdirect", 48, 32, 8, 12, rr_depth=3, engine=e))
+    (g, st), (m, st2) = out[ngi.ENGINE_PERSISTENT], out[ngi.ENGINE_MEGAKERNEL]
+    assert st["samples"] == st2["samples"] == 48 * 32 * 8
+    assert abs(st["extension_rays"] - st2["extension_rays"]) <= 1e-3 * st2["extension_rays"]
+    _crn_close(g, m, 2e-4)
+
+
 @pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
 def test_scene_without_triangles(width):
     """A camera and a point light and nothing else: every ray misses (an

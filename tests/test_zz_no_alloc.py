@@ -92,7 +92,11 @@ CASES = [("pt", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 1e-9),
          ("ptdirect", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC, 1e-9)]
 
 
-@pytest.mark.parametrize("kind,detail", [(ngi.GEN_CORNELL, 0), (ngi.GEN_SPHERE_FIELD, 4)])
+# Cornell box and the 13k-triangle sphere field: chunked path feed, shadow
+# rays in rounds of 64; the 29k-triangle sphere field: exact refills,
+# resumable traversals and shadow drains from the 128-record rings the
+# scratch array holds.
+@pytest.mark.parametrize("kind,detail", [(ngi.GEN_CORNELL, 0), (ngi.GEN_SPHERE_FIELD, 4), (ngi.GEN_SPHERE_FIELD, 5)])
 @pytest.mark.parametrize("renderer,engine,precision,rtol", CASES)
 def test_noalloc_matches_render_twice(kind, detail, renderer, engine, precision, rtol):
     dev = _Device()
