@@ -184,7 +184,7 @@ def test_autograd_harness_and_inverse_rendering_descends():
             refl.clamp_(0.01, 1.0)
         losses.append(float(loss.detach()))
     assert losses[-1] < 0.5 * losses[0], losses
-    assert float((refl - refl0)[rmask].abs().mean()) < 0.5 * float((0.5 - refl0[rmask]).abs().mean())
+    assert float((refl.detach() - refl0)[rmask].abs().mean()) < 0.5 * float((0.5 - refl0[rmask]).abs().mean())
 
 
 @pytest.mark.gpu
