@@ -80,6 +80,81 @@ __device__ __forceinline__ void grad_add(double* lds, double* glob, int nlds, in
     else unsafeAtomicAdd(glob + 3 * prim + c, (double)v);
 }
 
+// Adjoint sums of one path: SA / SB = weighted contributions with no / one
+// zero-reflectance vertex before them, zc = per-channel count of such
+// vertices (2 bits each, saturating at 2), nrec = vertex records written.
+struct AdjPath {
+    f3 SA, SB;
+    uint32_t zc;
+    int nrec;
+};
+
+// A contribution C at pixel `pix` from emitter primitive `emitter_prim`.
+template <bool ZEROS>
+__device__ __forceinline__ void adj_contribute(const AdjArgs& g, const DevPrim<float>* prims, double* sL, int nlds, AdjPath& q, int pix,
+                                               f3 C, int emitter_prim) {
+    const float* gp = g.dLdI + 3 * (size_t)pix;
+    const float gC[3] = {gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale};
+    const DevPrim<float>& E = prims[emitter_prim];
+    const uint32_t zm = ZEROS ? g.zmask[emitter_prim] : 0u;
+    float sa[3] = {q.SA.x, q.SA.y, q.SA.z}, sb[3] = {q.SB.x, q.SB.y, q.SB.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t z = (q.zc >> (2 * c)) & 3u;
+        const bool le_zero = (zm & (NGI_ADJ_LEZERO << c)) != 0;
+        if (z == 0) {
+            grad_add(sL, g.gLe, nlds, emitter_prim, c, gC[c] / E.Le[c]);   // Le~ = 1 where Le = 0
+            if (!le_zero) sa[c] += gC[c];
+        } else if (z == 1 && !le_zero) {
+            sb[c] += gC[c];
+        }
+    }
+    q.SA = f3(sa[0], sa[1], sa[2]);
+    q.SB = f3(sb[0], sb[1], sb[2]);
+}
+
+// The record of a surface vertex whose reflectance enters the path here.
+template <bool ZEROS>
+__device__ __forceinline__ void adj_record(const AdjArgs& g, uint32_t gid, AdjPath& q, int prim) {
+    const uint32_t zm = ZEROS ? g.zmask[prim] : 0u;
+    uint32_t w = (uint32_t)prim;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (!(zm & (NGI_ADJ_RZERO << c))) continue;
+        const uint32_t z = (q.zc >> (2 * c)) & 3u;
+        if (z == 0) w |= 1u << (29 + c);
+        if (z < 2) q.zc += 1u << (2 * c);
+    }
+    g.rec[(size_t)q.nrec * g.lanes + gid] = make_float4(q.SA.x, q.SA.y, q.SA.z, __uint_as_float(w));
+    q.nrec++;
+}
+
+// End of a path: the reflectance gradients of its records.
+template <bool ZEROS>
+__device__ __forceinline__ void adj_finish(const AdjArgs& g, const DevPrim<float>* prims, double* sR, int nlds, uint32_t gid, const AdjPath& q) {
+    const float sa[3] = {q.SA.x, q.SA.y, q.SA.z}, sb[3] = {q.SB.x, q.SB.y, q.SB.z};
+    for (int r = 0; r < q.nrec; ++r) {
+        const float4 e = g.rec[(size_t)r * g.lanes + gid];
+        const uint32_t w = __float_as_uint(e.w);
+        const int m = (int)(w & NGI_ADJ_PRIM_MASK);
+        const DevPrim<float>& P = prims[m];
+        const uint32_t zm = ZEROS ? g.zmask[m] : 0u;
+        const float pa[3] = {e.x, e.y, e.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!(zm & (NGI_ADJ_RZERO << c))) grad_add(sR, g.gR, nlds, m, c, (sa[c] - pa[c]) / P.R[c]);
+            else if (w & (1u << (29 + c))) grad_add(sR, g.gR, nlds, m, c, sb[c]);
+        }
+    }
+}
+
+// The light picked by a NEE sample (the draw of sample_light).
+__device__ __forceinline__ int nee_light(const DevScene& sc, const DevLight<float>* lights, U4 rn) {
+    int li = (int)(u24f(rn.x) * (float)sc.num_lights);
+    li = li < 0 ? 0 : (li > sc.num_lights - 1 ? sc.num_lights - 1 : li);
+    return lights[li].prim;
+}
+
 // ZEROS: some parameter channel is zero (the zero bookkeeping is compiled in
 // only then; without it every mask is 0 and z stays 0).
 template <bool DIRECT, int WIDTH, bool ZEROS>
@@ -101,52 +176,13 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
     const uint32_t gid = blockIdx.x * NGI_ADJ_BLOCK + threadIdx.x;
     uint32_t* my_stk = stk + threadIdx.x;
 
-    // Path state (as ProcessSample_PT/_PTDirect) plus the adjoint sums:
-    // SA / SB = weighted contributions with no / one zero-reflectance vertex
-    // before them, zc = per-channel count of such vertices (2 bits each,
-    // saturating at 2).
-    f3 T(0.0f), wi(0.0f), SA(0.0f), SB(0.0f);
+    // Path state (as ProcessSample_PT/_PTDirect) plus the adjoint sums.
+    f3 T(0.0f), wi(0.0f);
+    AdjPath q{f3(0.0f), f3(0.0f), 0u, 0};
     Surf<float> s;
-    int type = 0, prim = 0, nv = 0, pixel = 0, nrec = 0;
-    uint32_t c0 = 0, c1 = 0, zc = 0;
+    int type = 0, prim = 0, nv = 0, pixel = 0;
+    uint32_t c0 = 0, c1 = 0;
     bool active = false, exhausted = false;
-
-    auto contribute = [&](int pix, f3 C, int emitter_prim) {
-        const float* gp = g.dLdI + 3 * (size_t)pix;
-        const float gC[3] = {gp[0] * C.x * g.scale, gp[1] * C.y * g.scale, gp[2] * C.z * g.scale};
-        const DevPrim<float>& E = prims[emitter_prim];
-        const uint32_t zm = ZEROS ? g.zmask[emitter_prim] : 0u;
-        float sa[3] = {SA.x, SA.y, SA.z}, sb[3] = {SB.x, SB.y, SB.z};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const uint32_t z = (zc >> (2 * c)) & 3u;
-            const bool le_zero = (zm & (NGI_ADJ_LEZERO << c)) != 0;
-            if (z == 0) {
-                grad_add(sL, g.gLe, nlds, emitter_prim, c, gC[c] / E.Le[c]);   // Le~ = 1 where Le = 0
-                if (!le_zero) sa[c] += gC[c];
-            } else if (z == 1 && !le_zero) {
-                sb[c] += gC[c];
-            }
-        }
-        SA = f3(sa[0], sa[1], sa[2]);
-        SB = f3(sb[0], sb[1], sb[2]);
-    };
-    auto finish_path = [&]() {
-        const float sa[3] = {SA.x, SA.y, SA.z}, sb[3] = {SB.x, SB.y, SB.z};
-        for (int r = 0; r < nrec; ++r) {
-            const float4 e = g.rec[(size_t)r * g.lanes + gid];
-            const uint32_t w = __float_as_uint(e.w);
-            const int m = (int)(w & NGI_ADJ_PRIM_MASK);
-            const DevPrim<float>& P = prims[m];
-            const uint32_t zm = ZEROS ? g.zmask[m] : 0u;
-            const float pa[3] = {e.x, e.y, e.z};
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (!(zm & (NGI_ADJ_RZERO << c))) grad_add(sR, g.gR, nlds, m, c, (sa[c] - pa[c]) / P.R[c]);
-                else if (w & (1u << (29 + c))) grad_add(sR, g.gR, nlds, m, c, sb[c]);
-            }
-        }
-    };
 
     for (;;) {
         if (!exhausted) {
@@ -168,10 +204,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
                         s.degenerate = true;
                         wi = f3(0.0f);
                         nv = 1;
-                        SA = f3(0.0f);
-                        SB = f3(0.0f);
-                        zc = 0;
-                        nrec = 0;
+                        q = AdjPath{f3(0.0f), f3(0.0f), 0u, 0};
                         active = true;
                     }
                 }
@@ -181,20 +214,8 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
         if (!active) continue;
         // ---- one vertex (src/nanogi.cpp:483-606 / 645-801) ----
         bool alive = nv < max_nv;
-        if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && nrec < g.max_rec) {
-            // Vertex reflectance enters here (NEE fsE, then the BSDF weight).
-            const uint32_t zm = ZEROS ? g.zmask[prim] : 0u;
-            uint32_t w = (uint32_t)prim;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (!(zm & (NGI_ADJ_RZERO << c))) continue;
-                const uint32_t z = (zc >> (2 * c)) & 3u;
-                if (z == 0) w |= 1u << (29 + c);
-                if (z < 2) zc += 1u << (2 * c);
-            }
-            g.rec[(size_t)nrec * g.lanes + gid] = make_float4(SA.x, SA.y, SA.z, __uint_as_float(w));
-            nrec++;
-        }
+        // Vertex reflectance enters here (NEE fsE, then the BSDF weight).
+        if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && q.nrec < g.max_rec) adj_record<ZEROS>(g, gid, q, prim);
         if (alive && DIRECT && sc.num_lights > 0) {
             const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
             TraceRay sr;
@@ -203,12 +224,8 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
             if (!is_zero(C)) {
                 TraceHit sh;
                 TraceCounters tc{0, 0};
-                if (!trace<WIDTH, true, false>(sc, sr, sh, tc, my_stk, NGI_ADJ_BLOCK)) {
-                    // The light picked by the NEE sample (same draw as sample_light).
-                    int li = (int)(u24f(rn.x) * (float)sc.num_lights);
-                    li = li < 0 ? 0 : (li > sc.num_lights - 1 ? sc.num_lights - 1 : li);
-                    contribute(target, C, lights[li].prim);
-                }
+                if (!trace<WIDTH, true, false>(sc, sr, sh, tc, my_stk, NGI_ADJ_BLOCK))
+                    adj_contribute<ZEROS>(g, prims, sL, nlds, q, target, C, nee_light(sc, lights, rn));
             }
         }
         f3 wo(0.0f);
@@ -232,7 +249,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
                     const DevPrim<float>& hm = prims[hp];
                     if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA &&
                         dot(ns.n, -wo) > 0.0f)
-                        contribute(pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                        adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
                     if (a.rr_depth >= 0 && nv >= a.rr_depth) {
                         if (u24f(rx.w) > a.rr_prob_f) alive = false;
                         else T = T / a.rr_prob_f;
@@ -247,7 +264,7 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
             }
         }
         if (!alive) {
-            finish_path();
+            adj_finish<ZEROS>(g, prims, sR, nlds, gid, q);
             active = false;
         }
     }
@@ -258,21 +275,203 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
     }
 }
 
-template <bool DIRECT, int WIDTH, bool ZEROS>
+// The adjoint kernel with resumable extension traversals (wide BVH): a
+// wavefront stops traversing once at most NGI_ADJ_KEEP of its lanes are still
+// inside the tree; the others complete their vertices and start their next
+// rays while those wait, their traversal state, direction and roulette number
+// parked in LDS.  NEE shadow rays are traced where they are made (their
+// contributions enter the adjoint sums before the next vertex record), on a
+// second LDS stack.  Every path takes the same vertices as in adjoint_kernel.
+#ifndef NGI_ADJ_KEEP
+#define NGI_ADJ_KEEP 8
+#endif
+template <bool DIRECT, bool ZEROS>
+__global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_resume_kernel(RenderArgs a, AdjArgs g, unsigned long long* next) {
+    // LDS: the FP64 accumulators, then extension stacks, shadow stacks
+    // (a.sc.stack_entries words per lane each) and the parked lane state.
+    extern __shared__ double adj_smem[];
+    const int nlds = min(g.nprims, NGI_ADJ_LDS_PRIMS);
+    double* sR = adj_smem;
+    double* sL = adj_smem + 3 * nlds;
+    uint32_t* words = reinterpret_cast<uint32_t*>(adj_smem + 6 * nlds);
+    for (int i = threadIdx.x; i < 6 * nlds; i += NGI_ADJ_BLOCK) adj_smem[i] = 0.0;
+    __syncthreads();
+
+    const DevScene& sc = a.sc;
+    const Cam<float> cam = load_cam<float>(sc.cam);
+    const DevPrim<float>* prims = prims_of<float>(sc);
+    const DevLight<float>* lights = lights_of<float>(sc);
+    const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
+    const uint32_t lane = __lane_id();
+    const uint32_t gid = blockIdx.x * NGI_ADJ_BLOCK + threadIdx.x;
+    uint32_t* ext_stk = words + threadIdx.x;
+    uint32_t* sh_stk = words + (size_t)sc.stack_entries * NGI_ADJ_BLOCK + threadIdx.x;
+    uint32_t* pk = words + (size_t)2 * sc.stack_entries * NGI_ADJ_BLOCK + threadIdx.x;
+    const bool geometry = sc.num_tris != 0;
+
+    f3 T(0.0f), wi(0.0f);
+    AdjPath q{f3(0.0f), f3(0.0f), 0u, 0};
+    Surf<float> s;
+    int type = 0, prim = 0, nv = 0, pixel = 0;
+    uint32_t c0 = 0, c1 = 0;
+    bool active = false;      // the lane holds a path
+    bool busy = false;        // ... whose extension ray is being traced
+    bool exhausted = false;   // wave-uniform: no path ids left
+    for (;;) {
+        // Vertices whose extension rays have finished (hit, emission for pt,
+        // Russian roulette).
+        if (active && !busy) {
+            const TraceHit h = parked_hit<NGI_ADJ_BLOCK>(pk);
+            bool alive = h.leaf >= 0;
+            if (alive) {
+                const f3 wo = parked_dir<NGI_ADJ_BLOCK>(pk);
+                int hp;
+                const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp);
+                const DevPrim<float>& hm = prims[hp];
+                if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA && dot(ns.n, -wo) > 0.0f)
+                    adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                if (a.rr_depth >= 0 && nv >= a.rr_depth) {
+                    if (parked_urr<NGI_ADJ_BLOCK>(pk) > a.rr_prob_f) alive = false;
+                    else T = T / a.rr_prob_f;
+                }
+                type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                if (type == 0) alive = false;
+                s = ns;
+                prim = hp;
+                wi = -wo;
+                nv++;
+            }
+            if (!alive) {
+                adj_finish<ZEROS>(g, prims, sR, nlds, gid, q);
+                active = false;
+            }
+        }
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= a.path_count) exhausted = true;
+                if (!active) {
+                    const unsigned long long local = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    if (local < a.path_count) {
+                        path_coords(a, a.path_begin + local, c0, c1, pixel);
+                        T = f3(1.0f);
+                        type = NGI_PRIM_E;
+                        prim = sc.sensor_prim;
+                        s.p = cam.eye;
+                        s.degenerate = true;
+                        wi = f3(0.0f);
+                        nv = 1;
+                        q = AdjPath{f3(0.0f), f3(0.0f), 0u, 0};
+                        active = true;
+                    }
+                }
+            }
+        }
+        // Next vertex of every path not inside the tree: record, NEE,
+        // direction sample, extension ray.
+        if (active && !busy) {
+            bool alive = nv < max_nv;
+            if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && q.nrec < g.max_rec) adj_record<ZEROS>(g, gid, q, prim);
+            if (alive && DIRECT && sc.num_lights > 0) {
+                const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
+                TraceRay sr;
+                int target;
+                const f3 C = nee_contribution<float>(sc, cam, prims, type, prim, s, wi, T, rn, a.w, a.h, pixel, sr, target);
+                if (!is_zero(C)) {
+                    TraceHit sh;
+                    TraceCounters tc{0, 0};
+                    if (!trace<8, true, false>(sc, sr, sh, tc, sh_stk, NGI_ADJ_BLOCK))
+                        adj_contribute<ZEROS>(g, prims, sL, nlds, q, target, C, nee_light(sc, lights, rn));
+                }
+            }
+            if (alive) {
+                const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
+                f3 wo(0.0f);
+                if (type == NGI_PRIM_E) {
+                    wo = camera_sample<float>(a, cam, u24f(rx.x), u24f(rx.y), pixel);
+                } else {
+                    f3 wgt;
+                    alive = bsdf_sample<float>(prims[prim], type, s, wi, u24f(rx.x), u24f(rx.y), u24f(rx.z), wo, wgt);
+                    if (alive) T = T * wgt;
+                }
+                if (alive) {
+                    Trav8 tv;
+                    trav8_begin(FLT_MAX, tv);
+                    park_trav<NGI_ADJ_BLOCK>(pk, tv);
+                    park_dir<NGI_ADJ_BLOCK>(pk, wo, u24f(rx.w));
+                    busy = geometry;
+                }
+            }
+            if (!alive) {
+                adj_finish<ZEROS>(g, prims, sR, nlds, gid, q);
+                active = false;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (__ballot(busy)) {
+            // Lanes without a ray load and store back what they parked last.
+            Trav8 tv = parked_trav<NGI_ADJ_BLOCK>(pk);
+            const f3 d = parked_dir<NGI_ADJ_BLOCK>(pk);
+            const RayPre r = make_pre(TraceRay{s.p.x, s.p.y, s.p.z, d.x, d.y, d.z, NGI_EPS_F, FLT_MAX});
+            const int keep = __builtin_amdgcn_readfirstlane(exhausted ? 0 : NGI_ADJ_KEEP);
+            TraceCounters tc{0, 0};
+            trav8_run<false>(sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_ADJ_BLOCK);
+            park_trav<NGI_ADJ_BLOCK>(pk, tv);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * nlds; i += NGI_ADJ_BLOCK) {
+        if (sR[i] != 0.0) unsafeAtomicAdd(g.gR + i, sR[i]);
+        if (sL[i] != 0.0) unsafeAtomicAdd(g.gLe + i, sL[i]);
+    }
+}
+
+template <bool DIRECT, int WIDTH, bool ZEROS, bool RESUME>
+static const void* adjoint_fn() {
+    if (RESUME) return reinterpret_cast<const void*>(adjoint_resume_kernel<DIRECT, ZEROS>);
+    return reinterpret_cast<const void*>(adjoint_kernel<DIRECT, WIDTH, ZEROS>);
+}
+
+template <bool DIRECT, int WIDTH, bool ZEROS, bool RESUME>
 static int launch_adjoint(const RenderArgs& a, const AdjArgs& g, unsigned long long* next, unsigned grid, size_t lds, hipStream_t st) {
-    adjoint_kernel<DIRECT, WIDTH, ZEROS><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
+    if (RESUME) adjoint_resume_kernel<DIRECT, ZEROS><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
+    else adjoint_kernel<DIRECT, WIDTH, ZEROS><<<grid, NGI_ADJ_BLOCK, lds, st>>>(a, g, next);
     NGI_HIP_CHECK(hipGetLastError());
     return NGI_OK;
 }
 
-template <bool DIRECT, int WIDTH, bool ZEROS>
+template <bool DIRECT, int WIDTH, bool ZEROS, bool RESUME>
 static int adjoint_grid(size_t lds, unsigned& grid) {
     int dev = 0, cus = 0, per_cu = 0;
     NGI_HIP_CHECK(hipGetDevice(&dev));
     NGI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<DIRECT, WIDTH, ZEROS>, NGI_ADJ_BLOCK, lds));
+    NGI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_fn<DIRECT, WIDTH, ZEROS, RESUME>(), NGI_ADJ_BLOCK, lds));
     grid = (unsigned)std::max(1, cus * std::max(1, per_cu));
     return NGI_OK;
+}
+
+// One of the eight kernel variants, chosen at run time.
+template <bool RESUME>
+static int adjoint_dispatch(bool direct, int width, bool zeros, bool grid_only, const RenderArgs& a, const AdjArgs& g, unsigned long long* next,
+                            unsigned& grid, size_t lds, hipStream_t st) {
+#define NGI_ADJ_CASE(D, W, Z) \
+    return grid_only ? adjoint_grid<D, W, Z, RESUME>(lds, grid) : launch_adjoint<D, W, Z, RESUME>(a, g, next, grid, lds, st)
+    if (direct) {
+        if (width == 8) { if (zeros) NGI_ADJ_CASE(true, 8, true); else NGI_ADJ_CASE(true, 8, false); }
+        if (zeros) NGI_ADJ_CASE(true, 2, true);
+        NGI_ADJ_CASE(true, 2, false);
+    }
+    if (width == 8) { if (zeros) NGI_ADJ_CASE(false, 8, true); else NGI_ADJ_CASE(false, 8, false); }
+    if (zeros) NGI_ADJ_CASE(false, 2, true);
+    NGI_ADJ_CASE(false, 2, false);
+#undef NGI_ADJ_CASE
 }
 
 int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* dLdI, double* gR_out, double* gLe_out, ngi_stats* stats) {
@@ -299,7 +498,11 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     const bool direct = p.renderer == NGI_RENDERER_PTDIRECT;
     const int width = scene->dev->view.bvh_width;
     const int nlds = std::min(nprims, NGI_ADJ_LDS_PRIMS);
-    const size_t lds = (size_t)6 * nlds * 8 + (width == 8 ? (size_t)scene->dev->view.stack_entries * NGI_ADJ_BLOCK * 4 : 0);
+    // Resumable traversals (wide BVH) while their LDS leaves the kernel its
+    // occupancy of NGI_ADJ_WAVES blocks per CU.
+    const size_t resume_lds = (size_t)6 * nlds * 8 + ((size_t)2 * scene->dev->view.stack_entries + PK_WORDS) * NGI_ADJ_BLOCK * 4;
+    const bool resume = width == 8 && resume_lds * NGI_ADJ_WAVES <= NGI_CU_LDS_BYTES;
+    const size_t lds = resume ? resume_lds : (size_t)6 * nlds * 8 + (width == 8 ? (size_t)scene->dev->view.stack_entries * NGI_ADJ_BLOCK * 4 : 0);
     unsigned grid = 1;
     std::vector<void*> mem;
     auto cleanup = [&]() {
@@ -342,10 +545,9 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
         for (int c = 0; c < 3; ++c)
             if (zm[rl[l].prim] & (NGI_ADJ_LEZERO << c)) rl[l].Le[c] = 1.0f;
     const bool zeros = std::any_of(zm.begin(), zm.end(), [](uint8_t v) { return v != 0; });
-    st = direct ? (width == 8 ? (zeros ? adjoint_grid<true, 8, true>(lds, grid) : adjoint_grid<true, 8, false>(lds, grid))
-                              : (zeros ? adjoint_grid<true, 2, true>(lds, grid) : adjoint_grid<true, 2, false>(lds, grid)))
-                : (width == 8 ? (zeros ? adjoint_grid<false, 8, true>(lds, grid) : adjoint_grid<false, 8, false>(lds, grid))
-                              : (zeros ? adjoint_grid<false, 2, true>(lds, grid) : adjoint_grid<false, 2, false>(lds, grid)));
+    RenderArgs a = S.args(nullptr);
+    st = resume ? adjoint_dispatch<true>(direct, width, zeros, true, a, g, nullptr, grid, lds, str)
+                : adjoint_dispatch<false>(direct, width, zeros, true, a, g, nullptr, grid, lds, str);
     if (st) {
         cleanup();
         return st;
@@ -386,7 +588,6 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     g.gR = reinterpret_cast<double*>(d_gR);
     g.gLe = reinterpret_cast<double*>(d_gL);
     g.zmask = reinterpret_cast<const uint8_t*>(d_zm);
-    RenderArgs a = S.args(nullptr);
     a.sc.prims_f = reinterpret_cast<const DevPrim<float>*>(d_rp);
     a.sc.lights_f = reinterpret_cast<const DevLight<float>*>(d_rl);
     a.path_begin = 0;
@@ -394,12 +595,8 @@ int render_adjoint(ngi_scene* scene, const ngi_render_params& p, const double* d
     a.sample_base = 0;
     if (!st && total) {
         unsigned long long* nx = reinterpret_cast<unsigned long long*>(d_next);
-        if (zeros)
-            st = direct ? (width == 8 ? launch_adjoint<true, 8, true>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2, true>(a, g, nx, grid, lds, str))
-                        : (width == 8 ? launch_adjoint<false, 8, true>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2, true>(a, g, nx, grid, lds, str));
-        else
-            st = direct ? (width == 8 ? launch_adjoint<true, 8, false>(a, g, nx, grid, lds, str) : launch_adjoint<true, 2, false>(a, g, nx, grid, lds, str))
-                        : (width == 8 ? launch_adjoint<false, 8, false>(a, g, nx, grid, lds, str) : launch_adjoint<false, 2, false>(a, g, nx, grid, lds, str));
+        st = resume ? adjoint_dispatch<true>(direct, width, zeros, false, a, g, nx, grid, lds, str)
+                    : adjoint_dispatch<false>(direct, width, zeros, false, a, g, nx, grid, lds, str);
     }
     if (!st && hipMemcpyAsync(gR_out, d_gR, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");
     if (!st && hipMemcpyAsync(gLe_out, d_gL, (size_t)nprims * 3 * 8, hipMemcpyDeviceToHost, str) != hipSuccess) st = fail(NGI_ERR_DEVICE, "adjoint: read-back failed");

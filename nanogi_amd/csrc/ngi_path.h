@@ -509,6 +509,55 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
     return vertex_end<R, FilmT, DIRECT>(a, prims, ps, wo, urr, h);
 }
 
+// Lane state parked in LDS while a resumable traversal waits for the rest of
+// its wavefront to shade (the shading code needs the registers): traversal
+// state, direction and roulette number of the extension ray, one column per
+// lane, S words apart.
+enum { PK_CUR = 0, PK_MASK, PK_SP, PK_T, PK_U, PK_V, PK_LEAF, PK_WX, PK_WY, PK_WZ, PK_URR, PK_WORDS };
+#define NGI_CU_LDS_BYTES (160u * 1024u)   // LDS per CU on gfx950
+
+template <int S>
+__device__ __forceinline__ void park_trav(uint32_t* pk, const Trav8& t) {
+    pk[PK_CUR * S] = t.cur;
+    pk[PK_MASK * S] = t.mask;
+    pk[PK_SP * S] = (uint32_t)t.sp;
+    pk[PK_T * S] = __float_as_uint(t.h.t);
+    pk[PK_U * S] = __float_as_uint(t.h.u);
+    pk[PK_V * S] = __float_as_uint(t.h.v);
+    pk[PK_LEAF * S] = (uint32_t)t.h.leaf;
+}
+template <int S>
+__device__ __forceinline__ void park_dir(uint32_t* pk, V3<float> wo, float urr) {
+    pk[PK_WX * S] = __float_as_uint(wo.x);
+    pk[PK_WY * S] = __float_as_uint(wo.y);
+    pk[PK_WZ * S] = __float_as_uint(wo.z);
+    pk[PK_URR * S] = __float_as_uint(urr);
+}
+template <int S>
+__device__ __forceinline__ TraceHit parked_hit(const uint32_t* pk) {
+    TraceHit h;
+    h.t = __uint_as_float(pk[PK_T * S]);
+    h.u = __uint_as_float(pk[PK_U * S]);
+    h.v = __uint_as_float(pk[PK_V * S]);
+    h.leaf = (int32_t)pk[PK_LEAF * S];
+    return h;
+}
+template <int S>
+__device__ __forceinline__ Trav8 parked_trav(const uint32_t* pk) {
+    Trav8 t;
+    t.cur = pk[PK_CUR * S];
+    t.mask = pk[PK_MASK * S];
+    t.sp = (int)pk[PK_SP * S];
+    t.h = parked_hit<S>(pk);
+    return t;
+}
+template <int S>
+__device__ __forceinline__ V3<float> parked_dir(const uint32_t* pk) {
+    return V3<float>(__uint_as_float(pk[PK_WX * S]), __uint_as_float(pk[PK_WY * S]), __uint_as_float(pk[PK_WZ * S]));
+}
+template <int S>
+__device__ __forceinline__ float parked_urr(const uint32_t* pk) { return __uint_as_float(pk[PK_URR * S]); }
+
 __device__ __forceinline__ int clamp_vertices(int m) { return (m < 0 || m > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : m; }
 
 }  // namespace ngi

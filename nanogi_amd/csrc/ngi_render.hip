@@ -140,39 +140,6 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 #ifndef NGI_SHQ_LONG
 #define NGI_SHQ_LONG 512
 #endif
-enum { PK_CUR = 0, PK_MASK, PK_SP, PK_T, PK_U, PK_V, PK_LEAF, PK_WX, PK_WY, PK_WZ, PK_URR, PK_WORDS };
-#define NGI_CU_LDS_BYTES (160u * 1024u)
-
-__device__ __forceinline__ void park_trav(uint32_t* pk, const Trav8& t) {
-    pk[PK_CUR * NGI_MK_BLOCK] = t.cur;
-    pk[PK_MASK * NGI_MK_BLOCK] = t.mask;
-    pk[PK_SP * NGI_MK_BLOCK] = (uint32_t)t.sp;
-    pk[PK_T * NGI_MK_BLOCK] = __float_as_uint(t.h.t);
-    pk[PK_U * NGI_MK_BLOCK] = __float_as_uint(t.h.u);
-    pk[PK_V * NGI_MK_BLOCK] = __float_as_uint(t.h.v);
-    pk[PK_LEAF * NGI_MK_BLOCK] = (uint32_t)t.h.leaf;
-}
-__device__ __forceinline__ TraceHit parked_hit(const uint32_t* pk) {
-    TraceHit h;
-    h.t = __uint_as_float(pk[PK_T * NGI_MK_BLOCK]);
-    h.u = __uint_as_float(pk[PK_U * NGI_MK_BLOCK]);
-    h.v = __uint_as_float(pk[PK_V * NGI_MK_BLOCK]);
-    h.leaf = (int32_t)pk[PK_LEAF * NGI_MK_BLOCK];
-    return h;
-}
-__device__ __forceinline__ Trav8 parked_trav(const uint32_t* pk) {
-    Trav8 t;
-    t.cur = pk[PK_CUR * NGI_MK_BLOCK];
-    t.mask = pk[PK_MASK * NGI_MK_BLOCK];
-    t.sp = (int)pk[PK_SP * NGI_MK_BLOCK];
-    t.h = parked_hit(pk);
-    return t;
-}
-__device__ __forceinline__ V3<float> parked_dir(const uint32_t* pk) {
-    return V3<float>(__uint_as_float(pk[PK_WX * NGI_MK_BLOCK]), __uint_as_float(pk[PK_WY * NGI_MK_BLOCK]),
-                     __uint_as_float(pk[PK_WZ * NGI_MK_BLOCK]));
-}
-
 template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
 __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES) resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     using R = float;
@@ -198,7 +165,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
     for (;;) {
         // Vertices whose extension rays have finished.
         if (active && !busy)
-            active = vertex_end<R, FilmT, DIRECT>(a, prims, ps, parked_dir(pk), __uint_as_float(pk[PK_URR * NGI_MK_BLOCK]), parked_hit(pk));
+            active = vertex_end<R, FilmT, DIRECT>(a, prims, ps, parked_dir<NGI_MK_BLOCK>(pk), parked_urr<NGI_MK_BLOCK>(pk), parked_hit<NGI_MK_BLOCK>(pk));
         if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
             if (need) {
@@ -221,11 +188,8 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
                 Trav8 tv;
                 trav8_begin(FLT_MAX, tv);
                 busy = geometry;
-                park_trav(pk, tv);
-                pk[PK_WX * NGI_MK_BLOCK] = __float_as_uint(wo.x);
-                pk[PK_WY * NGI_MK_BLOCK] = __float_as_uint(wo.y);
-                pk[PK_WZ * NGI_MK_BLOCK] = __float_as_uint(wo.z);
-                pk[PK_URR * NGI_MK_BLOCK] = __float_as_uint(urr);
+                park_trav<NGI_MK_BLOCK>(pk, tv);
+                park_dir<NGI_MK_BLOCK>(pk, wo, urr);
             } else {
                 active = false;
             }
@@ -242,11 +206,11 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
         }
         if (__ballot(busy)) {
             // Lanes without a ray load and store back what they parked last.
-            Trav8 tv = parked_trav(pk);
-            const RayPre r = make_pre(extension_ray(ps, parked_dir(pk)));
+            Trav8 tv = parked_trav<NGI_MK_BLOCK>(pk);
+            const RayPre r = make_pre(extension_ray(ps, parked_dir<NGI_MK_BLOCK>(pk)));
             const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_PK_KEEP);
             trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_MK_BLOCK);
-            park_trav(pk, tv);
+            park_trav<NGI_MK_BLOCK>(pk, tv);
         }
     }
     if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;

@@ -159,6 +159,27 @@ def test_gpu_adjoint_matches_oracle_and_fd(renderer):
 
 
 @pytest.mark.gpu
+def test_gpu_adjoint_binary_and_wide_bvh_agree():
+    """The binary BVH replays paths with traversals run to the end by every
+    lane, the wide BVH with resumable traversals; the same paths give the
+    same gradients (both within the 2% of the FP64 oracle)."""
+    g = np.random.default_rng(6).normal(size=(24, 24, 3))
+    out = []
+    for width in (ngi.BVH_2, ngi.BVH_8):
+        s = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
+        s.build(width)
+        p = _params("ptdirect", 24, 24, 64, 8, 4)
+        p.engine = ngi.ENGINE_PERSISTENT
+        gR, gL, _ = ngi.render_adjoint(s, p, g)
+        out.append(np.concatenate([gR.ravel(), gL.ravel()]))
+    oR, oL = oracle.render_adjoint(s, p, g)
+    ref = np.concatenate([oR.ravel(), oL.ravel()])
+    assert np.linalg.norm(out[1] - out[0]) <= 1e-3 * np.linalg.norm(out[0])
+    for mine in out:
+        assert np.linalg.norm(mine - ref) <= 2e-2 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
 def test_autograd_harness_and_inverse_rendering_descends():
     torch = pytest.importorskip("torch")
     from nanogi_amd import autograd as nga
