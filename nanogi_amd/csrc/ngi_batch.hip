@@ -43,6 +43,7 @@ struct BatchDev {
     unsigned long long num_chunks;
     uint32_t chunk_paths;                          // paths per chunk (the last chunk of a scene may be shorter)
     int num_scenes;
+    uint32_t stack;                                // LDS traversal-stack entries per lane (the deepest scene's)
     ShadowQueue q;                                 // ptdirect: per-wave shadow-ray rings
 };
 
@@ -170,6 +171,142 @@ __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev 
     wave_sum_add(b.misc + MISC_SHADOW, cnt[CNT_SHADOW_RAYS]);
 }
 
+// The batch kernel with resumable extension traversals (wide BVHs), as the
+// single-scene resume_kernel: a wavefront stops traversing once at most
+// NGI_BATCH_KEEP of its lanes are still inside the tree, completes the
+// others' vertices and starts their next rays; the waiting lanes keep their
+// traversal state, direction and roulette number in LDS.  A wave leaves a
+// scene only with no path in flight, so no traversal outlives its scene.
+#ifndef NGI_BATCH_KEEP
+#define NGI_BATCH_KEEP 8
+#endif
+template <bool DIRECT, int BLOCK>
+__global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_resume_kernel(BatchDev b) {
+    // LDS: extension stacks, shadow stacks (b.stack words per lane each),
+    // then the parked lane state.
+    extern __shared__ uint32_t stk[];
+    uint32_t* ext_stk = stk + threadIdx.x;
+    uint32_t* sh_stk = stk + (size_t)b.stack * BLOCK + threadIdx.x;
+    uint32_t* pk = stk + (size_t)2 * b.stack * BLOCK + threadIdx.x;
+    if (threadIdx.x == 0) atomicMin(b.misc + MISC_T_START, (unsigned long long)wall_clock64());
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform shadow queue state (< NGI_SHQ)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    PathState<float> ps;
+    bool active = false;   // the lane holds a path
+    bool busy = false;     // ... whose extension ray is being traced
+    // Wave-uniform state.
+    int si = -1;                              // scene of the wave
+    unsigned long long next = 0, end = 0;     // unhanded paths of the current chunk (scene-local ids)
+    int pend_si = -1;                         // a chunk of another scene, taken while lanes were busy
+    unsigned long long pend_next = 0, pend_end = 0;
+    unsigned long long ended = 0;             // paths of scene si this wave finished, not yet counted
+    bool chunks_left = true;
+    for (;;) {
+        if (next >= end && chunks_left && pend_si < 0) {
+            unsigned long long k = 0;
+            if (lane == 0) k = atomicAdd(b.misc + MISC_CHUNK, 1ull);
+            k = __shfl(k, 0);
+            if (k >= b.num_chunks) {
+                chunks_left = false;
+            } else {
+                const int cs = owner(b.chunk, b.num_scenes, k);
+                const unsigned long long n = b.offset[cs + 1] - b.offset[cs];
+                const unsigned long long cb = (k - b.chunk[cs]) * (unsigned long long)b.chunk_paths;
+                const unsigned long long ce = cb + b.chunk_paths < n ? cb + b.chunk_paths : n;
+                if (si < 0 || cs == si) {
+                    si = cs;
+                    next = cb, end = ce;
+                } else {
+                    pend_si = cs;
+                    pend_next = cb, pend_end = ce;
+                }
+            }
+        }
+        const int su = __builtin_amdgcn_readfirstlane(si < 0 ? 0 : si);
+        const RenderArgs& a = b.args[su];
+        const DevPrim<float>* prims = prims_of<float>(a.sc);
+        // Vertices whose extension rays have finished.
+        bool ended_now = false;
+        if (active && !busy) {
+            active = vertex_end<float, double, DIRECT>(a, prims, ps, parked_dir<BLOCK>(pk), parked_urr<BLOCK>(pk), parked_hit<BLOCK>(pk));
+            ended_now = !active;
+        }
+        if (next < end) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const unsigned long long avail = end - next;
+                const unsigned long long rank = (unsigned long long)__popcll(need & below);
+                if (!active && rank < avail) {
+                    path_start<float>(a, next + rank, load_cam<float>(a.sc.cam), ps);
+                    active = true;
+                }
+                const unsigned long long took = (unsigned long long)__popcll(need);
+                next += took < avail ? took : avail;
+            }
+        }
+        // Next vertex of every path not inside the tree.
+        bool pushed = false;
+        if (active && !busy) {
+            V3<float> wo;
+            float urr;
+            if (vertex_begin<float, double, DIRECT, 8, false, DIRECT>(a, load_cam<float>(a.sc.cam), prims, ps, sh_stk, BLOCK,
+                                                                    clamp_vertices(a.max_nv), cnt, &b.q, wave, qhead + qcount, &pushed, wo,
+                                                                    urr)) {
+                cnt[CNT_EXT_RAYS]++;
+                Trav8 tv;
+                trav8_begin(FLT_MAX, tv);
+                busy = a.sc.num_tris != 0;
+                park_trav<BLOCK>(pk, tv);
+                park_dir<BLOCK>(pk, wo, urr);
+            } else {
+                active = false;
+                ended_now = true;
+            }
+        }
+        ended += (unsigned long long)__popcll(__ballot(ended_now));
+        const bool any = __ballot(active) != 0;
+        // All queued rays are traced before the wave leaves the scene.
+        if (DIRECT) {
+            qcount += (uint32_t)__popcll(__ballot(pushed));   // at most 63 + 64 < NGI_SHQ
+            shq_flush<double, 8, false>(a, b.q, wave, qhead, qcount, !any, sh_stk, BLOCK, cnt);
+        }
+        if (!any) {
+            // The wave has no path in flight: count what it finished on this
+            // scene, then move to the waiting chunk, the next chunk, or stop.
+            if (si >= 0 && ended) {
+                if (lane == 0) {
+                    const unsigned long long n = b.offset[su + 1] - b.offset[su];
+                    if (atomicAdd(b.done + su, ended) + ended == n) b.t_done[su] = (unsigned long long)wall_clock64();
+                }
+                ended = 0;
+            }
+            if (pend_si >= 0) {
+                si = pend_si;
+                next = pend_next, end = pend_end;
+                pend_si = -1;
+                continue;
+            }
+            if (next < end || chunks_left) continue;
+            break;
+        }
+        if (__ballot(busy)) {
+            // Lanes without a ray load and store back what they parked last.
+            Trav8 tv = parked_trav<BLOCK>(pk);
+            const RayPre r = make_pre(extension_ray(ps, parked_dir<BLOCK>(pk)));
+            // The last paths of the wave's chunk run their traversals to the end.
+            const int keep = __builtin_amdgcn_readfirstlane(next < end ? NGI_BATCH_KEEP : 0);
+            TraceCounters tc{0, 0};
+            trav8_run<false>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, BLOCK);
+            park_trav<BLOCK>(pk, tv);
+        }
+    }
+    wave_sum_add(b.misc + MISC_EXT, cnt[CNT_EXT_RAYS]);
+    wave_sum_add(b.misc + MISC_SHADOW, cnt[CNT_SHADOW_RAYS]);
+}
+
 // Clears the films and per-scene counters and resets the chunk counter: the
 // first node of the captured graph (a kernel, not a memset node).
 __global__ void batch_clear(double* film, size_t film_n, unsigned long long* done, unsigned long long* t_done, int n, unsigned long long* misc) {
@@ -183,18 +320,18 @@ __global__ void batch_clear(double* film, size_t film_n, unsigned long long* don
 using BatchKernel = void (*)(BatchDev);
 
 template <bool D, int W>
-static BatchKernel pick_block(int block) {
+static BatchKernel pick_block(int block, bool resume) {
     switch (block) {
-        case 64: return batch_kernel<D, W, 64>;
-        case 128: return batch_kernel<D, W, 128>;
-        case 256: return batch_kernel<D, W, 256>;
+        case 64: return resume && W == 8 ? batch_resume_kernel<D, 64> : batch_kernel<D, W, 64>;
+        case 128: return resume && W == 8 ? batch_resume_kernel<D, 128> : batch_kernel<D, W, 128>;
+        case 256: return resume && W == 8 ? batch_resume_kernel<D, 256> : batch_kernel<D, W, 256>;
         default: return nullptr;
     }
 }
 
-static BatchKernel pick_kernel(bool direct, int width, int block) {
-    if (direct) return width == 8 ? pick_block<true, 8>(block) : pick_block<true, 2>(block);
-    return width == 8 ? pick_block<false, 8>(block) : pick_block<false, 2>(block);
+static BatchKernel pick_kernel(bool direct, int width, int block, bool resume) {
+    if (direct) return width == 8 ? pick_block<true, 8>(block, resume) : pick_block<true, 2>(block, resume);
+    return width == 8 ? pick_block<false, 8>(block, resume) : pick_block<false, 2>(block, resume);
 }
 
 struct Batch {
@@ -243,14 +380,22 @@ struct Batch {
         b.num_chunks = nchunks;
         b.chunk_paths = c;
         b.num_scenes = n();
+        b.stack = stack;
         b.q.rec = shq;
         b.q.waves = 0;
         return b;
     }
-    size_t lds(int block) const { return width == 8 ? (size_t)stack * block * 4 : 0; }
+    // Resumable traversals (wide BVHs) while their LDS (two stacks and the
+    // parked state per lane) leaves NGI_BATCH_WAVES waves per SIMD.
+    bool resume() const { return width == 8 && ((size_t)2 * stack + PK_WORDS) * 4 * 256 * NGI_BATCH_WAVES <= NGI_CU_LDS_BYTES; }
+    size_t lds(int block) const {
+        if (width != 8) return 0;
+        return (resume() ? (size_t)2 * stack + PK_WORDS : (size_t)stack) * block * 4;
+    }
     int max_per_cu(int block) const {
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_kernel(direct, width, block), block, lds(block)) != hipSuccess) return 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pick_kernel(direct, width, block, resume()), block, lds(block)) != hipSuccess)
+            return 0;
         return occ;
     }
     int enqueue_clear() const {
@@ -263,7 +408,7 @@ struct Batch {
     int enqueue_render(const unsigned long long* off, const unsigned long long* chunk, unsigned long long nchunks, uint32_t c, int block,
                        int per_cu) const {
         if (!nchunks) return NGI_OK;
-        BatchKernel k = pick_kernel(direct, width, block);
+        BatchKernel k = pick_kernel(direct, width, block, resume());
         if (!k) return fail(NGI_ERR_INVALID_ARG, "batch block size %d is not one of 64, 128, 256", block);
         const unsigned grid = (unsigned)std::max(1, num_cus * per_cu);
         BatchDev d = dev(off, chunk, nchunks, c);
