@@ -407,19 +407,22 @@ def bench_nave(a):
     """The nave interior (about 77k triangles) with ptdirect at 1024x1024,
     `--nave-spp` samples per pixel, at most 16 segments, roulette from depth
     4: one render per engine (persistent, the plain megakernel, wavefront),
-    plus the FP64-shading megakernel and bdpt on 16-spp samples."""
+    plus the FP64-shading megakernel and bdpt on 16-spp samples and the
+    light tracer ltdirect on 64 light paths per pixel."""
     import nanogi_amd as ngi
     scene = ngi.Scene.generate(ngi.GEN_NAVE, 1, 1.0)
     scene.build(8)
     W = H = 1024
     spp = a.nave_spp
-    out = {"workload": f"ptdirect (and bdpt at 16 spp), nave ({scene.num_triangles} triangles), {W}x{H}, {spp} spp, max 16 segments, "
+    out = {"workload": f"ptdirect (and bdpt at 16 spp, ltdirect at 64 light paths per pixel), nave ({scene.num_triangles} triangles), "
+                       f"{W}x{H}, {spp} spp, max 16 segments, "
                        f"RR from depth 4", "engines": {}}
     variants = [("persistent", "ptdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, spp),
                 ("megakernel", "ptdirect", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC, spp),
                 ("wavefront", "ptdirect", ngi.ENGINE_WAVEFRONT, ngi.PRECISION_FP32_FP64ACC, spp),
                 ("megakernel_fp64", "ptdirect", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP64_ALL, min(spp, 16)),
-                ("bdpt", "bdpt", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, min(spp, 16))]
+                ("bdpt", "bdpt", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, min(spp, 16)),
+                ("ltdirect", "ltdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, min(spp, 64))]
     for name, renderer, eng, prec, n in variants:
         p = ngi.default_params(renderer=renderer, width=W, height=H, sampling=ngi.SAMPLING_PER_PIXEL, spp=n,
                                max_num_vertices=17, rr_depth=4, seed=1, engine=eng, precision=prec)

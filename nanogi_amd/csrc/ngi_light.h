@@ -77,13 +77,16 @@ __device__ __forceinline__ bool light_start(const RenderArgs& a, uint64_t id, Li
     return true;
 }
 
-// One iteration of the light-path loop.  Returns false when the path ends.
-// With QUEUE the sensor connection's shadow ray is appended to the
-// wavefront's shadow queue (as in path_vertex) instead of being traced here.
-template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
-__device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, LightState<R>& ls,
-                                             uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
-                                             uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
+// One iteration of the light-path loop in two halves around the extension
+// ray.  light_begin: the sensor connection (ltdirect) and the direction
+// sample; returns false when the path ends here, else the direction wo and
+// the roulette number urr.  With QUEUE the sensor connection's shadow ray is
+// appended to the wavefront's shadow queue (as in vertex_begin) instead of
+// being traced here.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE>
+__device__ __forceinline__ bool light_begin(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, LightState<R>& ls,
+                                            uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q, uint32_t qwave,
+                                            uint32_t qtail, bool* pushed, V3<R>& wo, R& urr) {
     const DevScene& sc = a.sc;
     if (ls.nv >= max_nv) return false;
     const DevLight<R>& L = lights_of<R>(sc)[ls.light];
@@ -138,8 +141,8 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
         }
     }
     const U4 rx = rng4(ls.c0, ls.c1, (uint32_t)ls.nv, NGI_STREAM_EXT, a.seed, a.stream_id);
-    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z), urr = unif<R>(rx.w);
-    V3<R> wo;
+    const R u0 = unif<R>(rx.x), u1 = unif<R>(rx.y), uc = unif<R>(rx.z);
+    urr = unif<R>(rx.w);
     if (ls.type == NGI_PRIM_L) {
         // Emitter SampleDirection / EvaluateDirection(.., LE, true) /
         // EvaluateDirectionPDF (rt.hpp:695-716, 917-937, 1154-1171).
@@ -159,16 +162,15 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
         if (!bsdf_sample<R, true>(prims[ls.prim], ls.type, ls.s, ls.wi, u0, u1, uc, wo, wgt)) return false;
         ls.T = ls.T * wgt;
     }
-    TraceRay er;
-    er.ox = (float)ls.s.p.x, er.oy = (float)ls.s.p.y, er.oz = (float)ls.s.p.z;
-    er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
-    er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
-    TraceHit h;
-    TraceCounters tc{0, 0};
-    const bool hit = trace<WIDTH, false, COUNT>(sc, er, h, tc, stk, stride);
-    cnt[CNT_EXT_RAYS]++;
-    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
-    if (!hit) return false;
+    return true;
+}
+
+// light_end: the hit of the extension ray (h.leaf = -1 for a miss) and
+// Russian roulette.  Returns false when the path ends.
+template <typename R>
+__device__ __forceinline__ bool light_end(const RenderArgs& a, const DevPrim<R>* prims, LightState<R>& ls, V3<R> wo, R urr, const TraceHit& h) {
+    const DevScene& sc = a.sc;
+    if (h.leaf < 0) return false;
     int hp;
     const Surf<R> ns = hit_surface<R>(sc, ls.s.p, wo, h, hp);
     if (a.rr_depth >= 0 && ls.nv >= a.rr_depth) {   // src/nanogi.cpp:910-921
@@ -183,6 +185,27 @@ __device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& 
     ls.wi = -wo;
     ls.nv++;
     return true;
+}
+
+// The whole light vertex with the extension ray traced in place.
+template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT, bool QUEUE = false>
+__device__ __forceinline__ bool light_vertex(const RenderArgs& a, const Cam<R>& cam, const DevPrim<R>* prims, LightState<R>& ls,
+                                             uint32_t* stk, int stride, int max_nv, uint32_t* cnt, const ShadowQueue* q = nullptr,
+                                             uint32_t qwave = 0, uint32_t qtail = 0, bool* pushed = nullptr) {
+    V3<R> wo;
+    R urr;
+    if (!light_begin<R, FilmT, DIRECT, WIDTH, COUNT, QUEUE>(a, cam, prims, ls, stk, stride, max_nv, cnt, q, qwave, qtail, pushed, wo, urr))
+        return false;
+    TraceRay er;
+    er.ox = (float)ls.s.p.x, er.oy = (float)ls.s.p.y, er.oz = (float)ls.s.p.z;
+    er.dx = (float)wo.x, er.dy = (float)wo.y, er.dz = (float)wo.z;
+    er.tmin = NGI_EPS_F, er.tmax = FLT_MAX;
+    TraceHit h;
+    TraceCounters tc{0, 0};
+    trace<WIDTH, false, COUNT>(a.sc, er, h, tc, stk, stride);
+    cnt[CNT_EXT_RAYS]++;
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    return light_end<R>(a, prims, ls, wo, urr, h);
 }
 
 }  // namespace ngi

@@ -74,10 +74,109 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
     }
 }
 
+// The light kernel for FP32 shading over the wide BVH with resumable
+// extension traversals, as resume_kernel for the path tracers: a wavefront
+// stops traversing once at most NGI_LT_KEEP of its lanes are still inside
+// the tree; the others complete their vertices, take new light paths and
+// start their next rays while the waiting lanes keep their traversal state,
+// direction and roulette number in LDS.  ltdirect's sensor connections go
+// through the shadow rings: drained with refills from rings of
+// NGI_SHQ_LONG records on large scenes, in rounds of 64 on small ones.
+#ifndef NGI_LT_KEEP
+#define NGI_LT_KEEP 8
+#endif
+template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
+__global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES) light_resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    using R = float;
+    constexpr bool QUEUE = DIRECT;
+    // LDS: extension stacks, shadow stacks (a.sc.stack_entries words per
+    // lane each), then the parked lane state.
+    extern __shared__ uint32_t stk[];
+    uint32_t* ext_stk = stk + threadIdx.x;
+    uint32_t* sh_stk = stk + (size_t)a.sc.stack_entries * NGI_LT_BLOCK + threadIdx.x;
+    uint32_t* pk = stk + (size_t)2 * a.sc.stack_entries * NGI_LT_BLOCK + threadIdx.x;
+    uint32_t cnt[CNT_N] = {0, 0, 0, 0, 0, 0, 0};
+    TraceCounters tc{0, 0};
+    const Cam<R> cam = load_cam<R>(a.sc.cam);
+    const DevPrim<R>* prims = prims_of<R>(a.sc);
+    const int max_nv = clamp_vertices(a.max_nv);
+    const uint32_t wave = blockIdx.x * (NGI_LT_BLOCK / 64) + (threadIdx.x >> 6);
+    const bool geometry = a.sc.num_tris != 0;
+    uint32_t qhead = 0, qcount = 0;   // wave-uniform: oldest slot, queued rays
+    LightState<R> ls;
+    PathFeed feed;
+    bool active = false;   // the lane holds a light path
+    bool busy = false;     // ... whose extension ray is being traced
+    for (;;) {
+        // Vertices whose extension rays have finished.
+        if (active && !busy) active = light_end<R>(a, prims, ls, parked_dir<NGI_LT_BLOCK>(pk), parked_urr<NGI_LT_BLOCK>(pk), parked_hit<NGI_LT_BLOCK>(pk));
+        if (!feed.done()) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
+                if (!active && id != ~0ull) {
+                    cnt[CNT_SAMPLES]++;
+                    active = light_start<R>(a, a.path_begin + id, ls);
+                }
+            }
+        }
+        // Next vertex of every light path not inside the tree.
+        bool pushed = false;
+        if (active && !busy) {
+            V3<R> wo;
+            R urr;
+            if (light_begin<R, FilmT, DIRECT, 8, COUNT, QUEUE>(a, cam, prims, ls, sh_stk, NGI_LT_BLOCK, max_nv, cnt, &q, wave, qhead + qcount,
+                                                               &pushed, wo, urr)) {
+                cnt[CNT_EXT_RAYS]++;
+                Trav8 tv;
+                trav8_begin(FLT_MAX, tv);
+                busy = geometry;
+                park_trav<NGI_LT_BLOCK>(pk, tv);
+                park_dir<NGI_LT_BLOCK>(pk, wo, urr);
+            } else {
+                active = false;
+            }
+        }
+        const bool any = __ballot(active) != 0;
+        if (QUEUE) {
+            qcount += (uint32_t)__popcll(__ballot(pushed));   // at most q.cap - 65 + 64 (63 + 64 chunked)
+            if (CHUNKED) shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
+            else shq_drain<FilmT, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
+        }
+        if (!any) {
+            if (feed.done()) break;
+            continue;
+        }
+        if (__ballot(busy)) {
+            // Lanes without a ray load and store back what they parked last.
+            Trav8 tv = parked_trav<NGI_LT_BLOCK>(pk);
+            const V3<R> d = parked_dir<NGI_LT_BLOCK>(pk);
+            const RayPre r = make_pre(TraceRay{ls.s.p.x, ls.s.p.y, ls.s.p.z, d.x, d.y, d.z, NGI_EPS_F, FLT_MAX});
+            const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_LT_KEEP);
+            trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_LT_BLOCK);
+            park_trav<NGI_LT_BLOCK>(pk, tv);
+        }
+    }
+    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+    if (a.counters) {
+        for (int k = 0; k < CNT_N; ++k) {
+            if (!COUNT && (k == CNT_EXT_NODES || k == CNT_EXT_TRIS || k == CNT_SH_NODES || k == CNT_SH_TRIS)) continue;
+            unsigned long long x = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if ((threadIdx.x & 63) == 0 && x) atomicAdd(a.counters + k, x);
+        }
+    }
+}
+
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
-    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4 : 0;
-    auto kern = a.path_chunk ? light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>;
+    // Resumable traversals while their LDS leaves the kernel its occupancy.
+    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + PK_WORDS) * NGI_LT_BLOCK * 4;
+    const unsigned blocks_per_cu = a.path_chunk && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES;
+    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
+    const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4) : 0;
+    auto kern = resume ? (a.path_chunk ? light_resume_kernel<FilmT, DIRECT, COUNT, true> : light_resume_kernel<FilmT, DIRECT, COUNT, false>)
+                       : (a.path_chunk ? light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>);
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_LT_BLOCK, lds, a.path_count, ln, &grid);
     if (s || !grid) return s;
@@ -85,7 +184,13 @@ static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     q.waves = grid * (NGI_LT_BLOCK / 64);
     q.rec = nullptr;
     if (DIRECT && std::is_same<R, float>::value) {
-        if ((s = mem.reserve((size_t)3 * q.waves * NGI_SHQ * sizeof(float4)))) return s;
+        // Longer rings for the refilling drains; a caller-owned buffer gets
+        // the longest ring it holds.
+        if (resume && !a.path_chunk) {
+            q.cap = NGI_SHQ_LONG;
+            while (mem.fixed && q.cap > NGI_SHQ && (size_t)3 * q.waves * q.cap * sizeof(float4) > *mem.bytes) q.cap >>= 1;
+        }
+        if ((s = mem.reserve((size_t)3 * q.waves * q.cap * sizeof(float4)))) return s;
         q.rec = reinterpret_cast<float4*>(*mem.ptr);
     }
     kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next, q);

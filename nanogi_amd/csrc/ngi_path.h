@@ -79,10 +79,19 @@ __device__ __forceinline__ bool trace(const DevScene& sc, const TraceRay& r, Tra
 // to its own ring of NGI_SHQ records in HBM and traces them 64 at a time.
 // The film receives the same contributions; only the order of the FP64
 // additions changes.  Record k of slot s of wavefront w is
-// rec[(k * waves + w) * NGI_SHQ + s]: (o, tmax), (d, C.r), (C.g, C.b, pixel, -).
+// rec[(k * waves + w) * cap + s]: (o, tmax), (d, C.r), (C.g, C.b, pixel, -).
 // A vertex step appends at most 64 rays to a queue holding at most 63.
 #ifndef NGI_SHQ
 #define NGI_SHQ 128
+#endif
+// Shadow rings of the resumable kernels on large scenes: NGI_SHQ_LONG records
+// per wavefront, drained with refills once they cannot take another 64 rays
+// (126 MB of rings at full occupancy).  Measured (M samples/s, sphere field /
+// nave): rounds of 64 from a 128-ring 1,638 / 682; drains of 128-ring 1,621 /
+// 663, 256: 1,721 / 704, 512: 1,737 / 717, 1024: 1,746 / 704.  Small scenes
+// (chunked path feed) keep rounds of 64: the drain cost 4% on the Cornell box.
+#ifndef NGI_SHQ_LONG
+#define NGI_SHQ_LONG 512
 #endif
 struct ShadowQueue {
     float4* rec;

@@ -131,15 +131,6 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 8
 #endif
-// Shadow rings of the resumable kernel on large scenes: NGI_SHQ_LONG records
-// per wavefront, drained with refills once they cannot take another 64 rays
-// (126 MB of rings at full occupancy).  Measured (M samples/s, sphere field /
-// nave): rounds of 64 from a 128-ring 1,638 / 682; drains of 128-ring 1,621 /
-// 663, 256: 1,721 / 704, 512: 1,737 / 717, 1024: 1,746 / 704.  Small scenes
-// (chunked path feed) keep rounds of 64: the drain cost 4% on the Cornell box.
-#ifndef NGI_SHQ_LONG
-#define NGI_SHQ_LONG 512
-#endif
 template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
 __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES) resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
     using R = float;
