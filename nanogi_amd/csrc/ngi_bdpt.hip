@@ -530,9 +530,103 @@ __device__ __forceinline__ void eval_strategy(const RenderArgs& a, const BdptCtx
     film_add<FilmT>(a.film, pixel_index(rx, ry, a.w, a.h), C);
 }
 
+// Subpath building blocks (bdpt.hpp:54-137): the first vertex of a side
+// (0: light, LE; 1: eye, EL), a direction sample from the current vertex,
+// and the vertex at a hit.
+__device__ __forceinline__ uint32_t bd_stream(int side) { return side == 0 ? NGI_STREAM_NEE : NGI_STREAM_EXT; }
+
+// The first vertex of `side`; false for the light side of a scene without lights.
+__device__ __forceinline__ bool bd_first(const RenderArgs& a, const BdptCtx& c, int side, uint32_t c0, uint32_t c1, BVert& pv) {
+    const DevScene& sc = *c.sc;
+    if (side == 0) {
+        if (sc.num_lights == 0) return false;
+        const U4 r0 = rng4(c0, c1, 0u, bd_stream(0), a.seed, a.stream_id);
+        const int li = pick_light(sc, u24f(r0.x));
+        const DevLight<float>& L = c.lights[li];
+        pv.prim = L.prim;
+        pv.type = NGI_PRIM_L;
+        if (L.type == NGI_LIGHT_AREA) {
+            area_light_point(sc, L, u24f(r0.y), u24f(r0.z), pv.p, pv.gn);
+            pv.n = pv.gn;
+            pv.deg = false;
+        } else if (L.type == NGI_LIGHT_POINT) {
+            pv.p = f3(L.pos[0], L.pos[1], L.pos[2]);
+            pv.gn = pv.n = f3(0.0f);
+            pv.deg = true;
+        } else {
+            const f3 d(L.dir[0], L.dir[1], L.dir[2]);
+            const Frame<float> fr = make_frame(d);
+            float dx, dy;
+            concentric_disk(u24f(r0.y), u24f(r0.z), dx, dy);
+            pv.p = f3(L.pos[0], L.pos[1], L.pos[2]) - d * L.radius + (fr.s * (dx * L.radius) + fr.t * (dy * L.radius));
+            pv.gn = pv.n = d;
+            pv.deg = false;
+        }
+        pv.alpha = f3(1.0f / pos_pdf(c, pv));   // EvaluatePosition(.., true) = 1
+    } else {
+        pv.prim = sc.sensor_prim;
+        pv.type = NGI_PRIM_E;
+        pv.p = c.cam.eye;
+        pv.gn = pv.n = f3(0.0f);
+        pv.deg = true;
+        pv.alpha = f3(1.0f);
+    }
+    return true;
+}
+
+// Direction sample at vertex `step - 1` of `side` (pv, with ppv_p the
+// vertex before it); false when the subpath ends there.
+__device__ __forceinline__ bool bd_direction(const RenderArgs& a, const BdptCtx& c, int side, int step, uint32_t c0, uint32_t c1, const BVert& pv,
+                                             f3 ppv_p, int& pixel, f3& wo, f3& wgt, float& urr) {
+    const U4 r = rng4(c0, c1, (uint32_t)step, bd_stream(side), a.seed, a.stream_id);
+    const float u0 = u24f(r.x), u1 = u24f(r.y), uc = u24f(r.z);
+    urr = u24f(r.w);
+    const f3 wi = step >= 2 ? vnorm(ppv_p - pv.p) : f3(0.0f);
+    if (pv.type == NGI_PRIM_L) {
+        const DevLight<float>& L = light_of(c, pv);
+        const f3 Le(L.Le[0], L.Le[1], L.Le[2]);
+        if (L.type == NGI_LIGHT_AREA) {
+            wo = to_world(make_frame(pv.n), cosine_hemisphere(u0, u1));
+            if (dot(pv.n, wo) <= 0.0f) return false;
+            wgt = Le * (float)NGI_PI;
+        } else if (L.type == NGI_LIGHT_POINT) {
+            wo = uniform_sphere(u0, u1);
+            wgt = Le * (float)(4.0 * NGI_PI);
+        } else {
+            wo = f3(L.dir[0], L.dir[1], L.dir[2]);
+            wgt = Le;
+        }
+        return true;
+    }
+    if (pv.type == NGI_PRIM_E) {
+        wo = camera_sample<float>(a, c.cam, u0, u1, pixel);
+        wgt = f3(1.0f);   // pinhole fs / pdf = 1
+        return true;
+    }
+    return side == 0 ? bsdf_sample<float, true>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt)
+                     : bsdf_sample<float, false>(c.prims[pv.prim], pv.type, surf_of(pv), wi, u0, u1, uc, wo, wgt);
+}
+
+// The vertex where the ray from pv along wo hit (h.leaf >= 0); alpha is the
+// subpath throughput up to it.
+__device__ __forceinline__ BVert bd_vertex(const BdptCtx& c, const BVert& pv, f3 wo, const TraceHit& h, f3 alpha) {
+    int hp;
+    const Surf<float> ns = hit_surface<float>(*c.sc, pv.p, wo, h, hp);
+    BVert v;
+    v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
+    v.prim = hp;
+    v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    v.alpha = alpha;
+    return v;
+}
+
 // QUEUE (wide BVH, scenes of at least NGI_CHUNKED_FEED_TRIS triangles): the
-// connection rays go through drained rings.  Measured (M samples/s, Cornell
-// box / nave at 16 spp): traced in place 198 / 64.1, queued 191 / 79.8.
+// subpaths' traversals are resumable and the connection rays go through
+// drained rings.  Measured (M samples/s, Cornell box / nave at 16 spp):
+// traced in place 198 / 64.1, connection rays queued 191 / 79.8.
+#ifndef NGI_BD_KEEP
+#define NGI_BD_KEEP 8
+#endif
 template <typename FilmT, int WIDTH, bool COUNT, bool QUEUE>
 __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(RenderArgs a, unsigned long long* next, BdptMem mem) {
     extern __shared__ uint32_t stk[];   // a.sc.stack_entries x block (wide BVH only)
@@ -565,7 +659,101 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
             break;
         }
         int nsub[2] = {0, 0};
-        if (have) {
+        if (QUEUE) {
+            // Subpaths with resumable traversals: every lane steps through its
+            // light subpath, then its eye subpath; a wavefront stops
+            // traversing once at most NGI_BD_KEEP lanes are inside the tree,
+            // and the others store their vertices and start their next rays.
+            uint32_t c0 = 0, c1 = 0;
+            int pixel = 0;
+            if (have) {
+                path_coords(a, a.path_begin + local, c0, c1, pixel);
+                cnt[CNT_SAMPLES]++;
+            }
+            uint32_t* pk = stk + (size_t)sc.stack_entries * NGI_BD_BLOCK + threadIdx.x;
+            bool done = !have, busy = false, traced = false;
+            int side = -1, step = 0, nv = 0;
+            BVert pv;
+            f3 ppv_p(0.0f);
+            for (;;) {
+                if (!done && !busy) {
+                    bool side_over = side < 0;
+                    if (traced) {   // the ray from vertex step - 1 has finished
+                        traced = false;
+                        const TraceHit h = parked_hit<NGI_BD_BLOCK>(pk);
+                        if (h.leaf < 0) {
+                            side_over = true;
+                        } else {
+                            const BVert v = bd_vertex(c, pv, parked_dir<NGI_BD_BLOCK>(pk), h, pv.alpha);
+                            bstore(mem, side, step, gl, v);
+                            nv = step + 1;
+                            const float q = rr_at(a, step);
+                            if (q < 1.0f && parked_urr<NGI_BD_BLOCK>(pk) > q) {
+                                side_over = true;   // the vertex is kept (bdpt.hpp:118-123)
+                            } else {
+                                ppv_p = pv.p;
+                                pv = v;
+                                ++step;
+                            }
+                        }
+                    }
+                    for (;;) {   // close a finished side, open the next, sample a direction
+                        if (side_over) {
+                            if (side >= 0) nsub[side] = nv;
+                            if (++side > 1) {
+                                done = true;
+                                break;
+                            }
+                            side_over = false;
+                            nv = 0;
+                            if (!bd_first(a, c, side, c0, c1, pv)) {
+                                side_over = true;
+                                continue;
+                            }
+                            bstore(mem, side, 0, gl, pv);
+                            ppv_p = f3(0.0f);
+                            nv = 1;
+                            step = 1;
+                        }
+                        f3 wo, wgt;
+                        float urr;
+                        if (step >= maxv || !bd_direction(a, c, side, step, c0, c1, pv, ppv_p, pixel, wo, wgt, urr)) {
+                            side_over = true;
+                            continue;
+                        }
+                        pv.alpha = pv.alpha * wgt;   // pv's record is already stored
+                        cnt[CNT_EXT_RAYS]++;
+                        Trav8 tv;
+                        trav8_begin(FLT_MAX, tv);
+                        park_trav<NGI_BD_BLOCK>(pk, tv);
+                        park_dir<NGI_BD_BLOCK>(pk, wo, urr);
+                        busy = sc.num_tris != 0;
+                        traced = true;
+                        break;
+                    }
+                }
+                const uint32_t working = (uint32_t)__popcll(__ballot(!done));
+                if (working == 0) break;
+                if (__ballot(busy)) {
+                    Trav8 tv = parked_trav<NGI_BD_BLOCK>(pk);
+                    const f3 d = parked_dir<NGI_BD_BLOCK>(pk);
+                    const RayPre r = make_pre(TraceRay{pv.p.x, pv.p.y, pv.p.z, d.x, d.y, d.z, NGI_EPS_F, FLT_MAX});
+                    // With few subpaths left, their traversals run to the end.
+                    const int keep = __builtin_amdgcn_readfirstlane(working > 2u * NGI_BD_KEEP ? NGI_BD_KEEP : 0);
+                    TraceCounters tc{0, 0};
+                    trav8_run<COUNT>(sc, r, NGI_EPS_F, tv, busy, keep, tc, my_stk, NGI_BD_BLOCK);
+                    if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
+                    park_trav<NGI_BD_BLOCK>(pk, tv);
+                }
+            }
+            if (have) {
+                subpath_factors(c, mem, 0, nsub[0], gl);
+                subpath_factors(c, mem, 1, nsub[1], gl);
+            }
+        } else if (have) {
+            // Small scenes: the subpaths traced vertex by vertex, written out
+            // here rather than through bd_first / bd_direction / bd_vertex,
+            // whose register allocation in this loop cost 6% on the Cornell box.
             uint32_t c0, c1;
             int pixel;
             path_coords(a, a.path_begin + local, c0, c1, pixel);
@@ -732,8 +920,9 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
 
 template <typename FilmT, int WIDTH, bool COUNT>
 static int launch_bd(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
-    const size_t lds = WIDTH == 8 ? (size_t)a.sc.stack_entries * NGI_BD_BLOCK * 4 : 0;
     const bool queue = WIDTH == 8 && a.sc.num_tris >= NGI_CHUNKED_FEED_TRIS;
+    // The resumable subpaths park their lane state next to the stack.
+    const size_t lds = WIDTH == 8 ? ((size_t)a.sc.stack_entries + (queue ? PK_WORDS : 0)) * NGI_BD_BLOCK * 4 : 0;
     auto kern = queue ? bdpt_kernel<FilmT, WIDTH, COUNT, WIDTH == 8> : bdpt_kernel<FilmT, WIDTH, COUNT, false>;
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_BD_BLOCK, lds, a.path_count, ln, &grid);
