@@ -177,8 +177,10 @@ __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_kernel(BatchDev 
 // others' vertices and starts their next rays; the waiting lanes keep their
 // traversal state, direction and roulette number in LDS.  A wave leaves a
 // scene only with no path in flight, so no traversal outlives its scene.
+// Measured on the 256 rooms (M samples/s): keep 4: 1,791, 8: 1,838,
+// 16: 1,866, 24: 1,858, 32: 1,833.
 #ifndef NGI_BATCH_KEEP
-#define NGI_BATCH_KEEP 8
+#define NGI_BATCH_KEEP 16
 #endif
 template <bool DIRECT, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, NGI_BATCH_WAVES) batch_resume_kernel(BatchDev b) {
