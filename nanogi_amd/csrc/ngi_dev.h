@@ -8,6 +8,7 @@
 //   lights_f/_d: DevLight<T> per light primitive (Scene::LightPrimitiveIndices)
 //   light_tris : 3 x float4 per area-light triangle, per light contiguous
 //   cdf_f/_d   : per area light, cdf[0..n] (rt.hpp:1747-1765)
+//   guide_f/_d : per area light, 2^guide_bits + 1 bucket starts of cdf_f/_d
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,6 +48,8 @@ struct DevLight {
     uint32_t tri_begin;  // first row in light_tris
     uint32_t tri_count;
     uint32_t cdf_begin;  // first entry in cdf (tri_count + 1 entries)
+    uint32_t guide_begin;  // first entry in guide (2^guide_bits + 1 entries)
+    uint32_t guide_bits;
     uint32_t pad;
     T inv_area;       // area light: 1 / area; directional: the reference's 1 / (2 pi r^2)
     T Le[3];
@@ -71,6 +74,8 @@ struct DevScene {
     const float4* light_tris = nullptr;
     const float* cdf_f = nullptr;
     const double* cdf_d = nullptr;
+    const uint32_t* guide_f = nullptr;
+    const uint32_t* guide_d = nullptr;
     int num_lights = 0;
     int bvh_width = 2;
     uint32_t stack_entries = 0;   // LDS traversal-stack entries per lane (wide BVH depth + 1)

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "ngi_dev.h"
+#include "ngi_guide.h"
 #include "ngi_traverse.h"
 
 namespace ngi {
@@ -49,6 +50,13 @@ static void fill_prim(const ngi_primitive& p, int light, DevPrim<T>& o) {
     o.s_eta2 = (T)p.S_eta2;
 }
 
+// Guide bits of an area light's triangle CDF (FP64; its FP32 rounding is
+// sorted and NaN-free whenever the FP64 CDF is).
+static uint32_t light_guide_bits(const std::vector<double>& cdf) {
+    const uint32_t m = (uint32_t)cdf.size();
+    return m >= 2 && guide_sorted(cdf.data(), m) ? guide_bits(m - 1) : 0u;
+}
+
 // Wait for every kernel that may read the scene's device arrays: the
 // scene's stream and the streams of the batches holding it.
 static int wait_users(HostScene& s) {
@@ -70,7 +78,7 @@ static int upload_params(HostScene& s, DeviceScene& d) {
     }
     std::vector<DevLight<float>> lf(s.light_prims.size());
     std::vector<DevLight<double>> ld(s.light_prims.size());
-    uint32_t tri_row = 0, cdf_row = 0;
+    uint32_t tri_row = 0, cdf_row = 0, guide_row = 0;
     for (size_t i = 0; i < s.light_prims.size(); ++i) {
         const int pi = s.light_prims[i];
         const ngi_primitive& p = s.prims[pi];
@@ -88,13 +96,17 @@ static int upload_params(HostScene& s, DeviceScene& d) {
             L.tri_begin = tri_row;
             L.tri_count = p.tri_count;
             L.cdf_begin = cdf_row;
+            L.guide_begin = guide_row;
+            L.guide_bits = light_guide_bits(s.area_dist[pi].cdf);
             L.inv_area = s.area_dist[pi].inv_area;
             tri_row += p.tri_count;
             cdf_row += p.tri_count + 1;
+            guide_row += (1u << L.guide_bits) + 1u;
         }
         ld[i] = L;
         DevLight<float>& F = lf[i];
-        F.prim = L.prim, F.type = L.type, F.tri_begin = L.tri_begin, F.tri_count = L.tri_count, F.cdf_begin = L.cdf_begin, F.pad = 0;
+        F.prim = L.prim, F.type = L.type, F.tri_begin = L.tri_begin, F.tri_count = L.tri_count, F.cdf_begin = L.cdf_begin;
+        F.guide_begin = L.guide_begin, F.guide_bits = L.guide_bits, F.pad = 0;
         F.inv_area = (float)L.inv_area;
         for (int k = 0; k < 3; ++k) F.Le[k] = (float)L.Le[k], F.pos[k] = (float)L.pos[k], F.dir[k] = (float)L.dir[k];
         F.radius = (float)L.radius;
@@ -178,6 +190,7 @@ int device_prepare(HostScene& s) {
     std::vector<float4> ltris;
     std::vector<float> cf;
     std::vector<double> cd;
+    std::vector<uint32_t> gf, gd;
     for (int pi : s.light_prims) {
         const ngi_primitive& p = s.prims[pi];
         if (p.light_type != NGI_LIGHT_AREA) continue;
@@ -187,14 +200,24 @@ int device_prepare(HostScene& s) {
                 ltris.push_back(make_float4((float)s.positions[3 * (size_t)vi], (float)s.positions[3 * (size_t)vi + 1],
                                             (float)s.positions[3 * (size_t)vi + 2], 0.0f));
             }
-        for (double c : s.area_dist[pi].cdf) {
-            cd.push_back(c);
-            cf.push_back((float)c);
+        const std::vector<double>& c = s.area_dist[pi].cdf;
+        const size_t f0 = cf.size();
+        for (double x : c) {
+            cd.push_back(x);
+            cf.push_back((float)x);
         }
+        // The guides follow upload_params' order and light_guide_bits.
+        const uint32_t bits = light_guide_bits(c), g0 = (uint32_t)gf.size();
+        gf.resize(g0 + (1u << bits) + 1u);
+        gd.resize(gf.size());
+        guide_build(cf.data() + f0, (uint32_t)c.size(), bits, gf.data() + g0);
+        guide_build(c.data(), (uint32_t)c.size(), bits, gd.data() + g0);
     }
     if ((st = upload(*d, ltris.data(), ltris.size(), &v.light_tris))) return st;
     if ((st = upload(*d, cf.data(), cf.size(), &v.cdf_f))) return st;
     if ((st = upload(*d, cd.data(), cd.size(), &v.cdf_d))) return st;
+    if ((st = upload(*d, gf.data(), gf.size(), &v.guide_f))) return st;
+    if ((st = upload(*d, gd.data(), gd.size(), &v.guide_d))) return st;
     // Pinhole camera (rt.hpp:1882-1902).
     memset(&v.cam, 0, sizeof(v.cam));
     if (s.sensor_prim >= 0 && s.prims[s.sensor_prim].sensor_type == NGI_SENSOR_PINHOLE) {

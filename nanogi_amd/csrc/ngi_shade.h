@@ -18,6 +18,7 @@
 #pragma once
 
 #include "ngi_dev.h"
+#include "ngi_guide.h"
 
 namespace ngi {
 
@@ -384,23 +385,22 @@ template <> __device__ __forceinline__ const DevLight<double>* lights_of<double>
 template <typename R> __device__ __forceinline__ const R* cdf_of(const DevScene& sc);
 template <> __device__ __forceinline__ const float* cdf_of<float>(const DevScene& sc) { return sc.cdf_f; }
 template <> __device__ __forceinline__ const double* cdf_of<double>(const DevScene& sc) { return sc.cdf_d; }
+template <typename R> __device__ __forceinline__ const uint32_t* guide_of(const DevScene& sc);
+template <> __device__ __forceinline__ const uint32_t* guide_of<float>(const DevScene& sc) { return sc.guide_f; }
+template <> __device__ __forceinline__ const uint32_t* guide_of<double>(const DevScene& sc) { return sc.guide_d; }
 template <typename R> __device__ __forceinline__ const DevPrim<R>* prims_of(const DevScene& sc);
 template <> __device__ __forceinline__ const DevPrim<float>* prims_of<float>(const DevScene& sc) { return sc.prims_f; }
 template <> __device__ __forceinline__ const DevPrim<double>* prims_of<double>(const DevScene& sc) { return sc.prims_d; }
 
 // A point on an area light: triangle by Distribution1D::SampleReuse
 // (basic.hpp:469-475: upper_bound - 1, clamped) over the light's area CDF,
+// searched through its guide table (the same index as the whole-CDF search),
 // then UniformSampleTriangle (rt.hpp:129-133) with the reused coordinate;
 // n = the triangle's winding normal (SampleTriangleMesh, rt.hpp:487-524).
 template <typename R>
 __device__ __forceinline__ void area_light_point(const DevScene& sc, const DevLight<R>& L, R u2x, R u2y, V3<R>& p, V3<R>& n) {
     const R* cdf = cdf_of<R>(sc) + L.cdf_begin;
-    uint32_t lo = 0, hi = L.tri_count + 1;   // search [lo, hi) for the first cdf > u
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cdf[mid] > u2x) hi = mid;
-        else lo = mid + 1;
-    }
+    const uint32_t lo = guided_upper_bound(cdf, guide_of<R>(sc) + L.guide_begin, L.guide_bits, u2x);
     int i = (int)lo - 1;
     i = i < 0 ? 0 : (i > (int)L.tri_count - 1 ? (int)L.tri_count - 1 : i);
     const R c0 = cdf[i], c1 = cdf[i + 1];
