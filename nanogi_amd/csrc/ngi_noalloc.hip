@@ -12,7 +12,10 @@
 //   [0, 256)             CNT_N 64-bit ray/sample counters, then the 64-bit
 //                        path counter of the persistent kernels
 //   [256, 256 + queues)  shadow-ray rings: NGI_NOALLOC_WAVES wavefronts x
-//                        NGI_SHQ records x 48 B (ptdirect / ltdirect, FP32)
+//                        NGI_SHQ records x 48 B (ptdirect / ltdirect, FP32
+//                        shading); FP64 shading's 64-B records fill it with
+//                        3,840 wavefronts, and its launches take as many
+//                        blocks as have rings (960 of 1,024 at 4 per CU)
 // Every call clears the counters with a fill kernel on the scene's stream,
 // so a call sees nothing of the previous one; calls that overlap on one GPU
 // (other streams, other threads) share the array and race.

@@ -10,6 +10,8 @@
 
 #include <float.h>
 
+#include <type_traits>
+
 #include "ngi_shade.h"
 #include "ngi_traverse.h"
 
@@ -107,35 +109,66 @@ __device__ __forceinline__ void shq_push(const ShadowQueue& q, uint32_t wave, ui
     *shq_at(q, 1, wave, slot) = make_float4(r.dx, r.dy, r.dz, C.x);
     *shq_at(q, 2, wave, slot) = make_float4(C.y, C.z, __int_as_float(pixel), 0.0f);
 }
-// Lanes 0..n-1 trace the n oldest shadow rays from `head` and splat the
-// unoccluded ones (Scene::Visible, rt.hpp:2251-2261; src/nanogi.cpp:700-706).
-template <typename FilmT, int WIDTH, bool COUNT>
-__device__ __forceinline__ void shq_trace(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t head, uint32_t n,
-                                          uint32_t* stk, int stride, uint32_t* cnt) {
-    if (__lane_id() >= n) return;
-    const uint32_t slot = head + __lane_id();
+// FP64 shading queues its FP32 shadow rays with FP64 contributions: four
+// float4s per record, (o, tmax), (d, pixel), (C.r, C.g), (C.b, -) at
+// rec[(k * waves + w) * cap + s], k = 0..3.
+template <typename CT> struct ShqWords { enum { value = 3 }; };
+template <> struct ShqWords<double> { enum { value = 4 }; };
+__device__ __forceinline__ float4 dbl2_bits(double x, double y) {
+    return make_float4(__int_as_float(__double2loint(x)), __int_as_float(__double2hiint(x)), __int_as_float(__double2loint(y)),
+                       __int_as_float(__double2hiint(y)));
+}
+__device__ __forceinline__ void shq_push(const ShadowQueue& q, uint32_t wave, uint32_t slot, const TraceRay& r, V3<double> C, int pixel) {
+    *shq_at(q, 0, wave, slot) = make_float4(r.ox, r.oy, r.oz, r.tmax);
+    *shq_at(q, 1, wave, slot) = make_float4(r.dx, r.dy, r.dz, __int_as_float(pixel));
+    *shq_at(q, 2, wave, slot) = dbl2_bits(C.x, C.y);
+    *shq_at(q, 3, wave, slot) = dbl2_bits(C.z, 0.0);
+}
+// The ray (tmin = EpsF), contribution and pixel of record `slot`.
+template <typename CT>
+__device__ __forceinline__ void shq_load(const ShadowQueue& q, uint32_t wave, uint32_t slot, TraceRay& sr, V3<CT>& C, int& pixel) {
     const float4 r0 = *shq_at(q, 0, wave, slot), r1 = *shq_at(q, 1, wave, slot), r2 = *shq_at(q, 2, wave, slot);
-    TraceRay sr;
     sr.ox = r0.x, sr.oy = r0.y, sr.oz = r0.z, sr.tmax = r0.w;
     sr.dx = r1.x, sr.dy = r1.y, sr.dz = r1.z;
     sr.tmin = NGI_EPS_F;
+    if constexpr (std::is_same<CT, double>::value) {
+        const float4 r3 = *shq_at(q, 3, wave, slot);
+        pixel = __float_as_int(r1.w);
+        C = V3<double>(__hiloint2double(__float_as_int(r2.y), __float_as_int(r2.x)),
+                       __hiloint2double(__float_as_int(r2.w), __float_as_int(r2.z)),
+                       __hiloint2double(__float_as_int(r3.y), __float_as_int(r3.x)));
+    } else {
+        pixel = __float_as_int(r2.z);
+        C = V3<float>(r1.w, r2.x, r2.y);
+    }
+}
+// Lanes 0..n-1 trace the n oldest shadow rays from `head` and splat the
+// unoccluded ones (Scene::Visible, rt.hpp:2251-2261; src/nanogi.cpp:700-706).
+template <typename FilmT, int WIDTH, bool COUNT, typename CT = float>
+__device__ __forceinline__ void shq_trace(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t head, uint32_t n,
+                                          uint32_t* stk, int stride, uint32_t* cnt) {
+    if (__lane_id() >= n) return;
+    TraceRay sr;
+    V3<CT> C;
+    int pixel;
+    shq_load<CT>(q, wave, head + __lane_id(), sr, C, pixel);
     TraceHit sh;
     TraceCounters tc{0, 0};
     const bool occluded = trace<WIDTH, true, COUNT>(a.sc, sr, sh, tc, stk, stride);
     cnt[CNT_SHADOW_RAYS]++;
     if (COUNT) cnt[CNT_SH_NODES] += tc.nodes, cnt[CNT_SH_TRIS] += tc.tris;
-    if (!occluded) film_add<FilmT>(a.film, __float_as_int(r2.z), V3<float>(r1.w, r2.x, r2.y));
+    if (!occluded) film_add<FilmT>(a.film, pixel, C);
 }
 
 // Traces the queued shadow rays in rounds of 64 (one ray per lane) while 64
 // or more are waiting, or all of them when `all` (the wave is about to stop
 // or to change scene).
-template <typename FilmT, int WIDTH, bool COUNT>
+template <typename FilmT, int WIDTH, bool COUNT, typename CT = float>
 __device__ __forceinline__ void shq_flush(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t& head, uint32_t& count,
                                           bool all, uint32_t* stk, int stride, uint32_t* cnt) {
     while (count >= 64u || (all && count > 0u)) {
         const uint32_t n = count < 64u ? count : 64u;
-        shq_trace<FilmT, WIDTH, COUNT>(a, q, wave, head, n, stk, stride, cnt);
+        shq_trace<FilmT, WIDTH, COUNT, CT>(a, q, wave, head, n, stk, stride, cnt);
         head += n;
         count -= n;
     }
@@ -151,7 +184,7 @@ __device__ __forceinline__ void shq_flush(const RenderArgs& a, const ShadowQueue
 #ifndef NGI_SHQ_REFILL
 #define NGI_SHQ_REFILL 16
 #endif
-template <typename FilmT, bool COUNT>
+template <typename FilmT, bool COUNT, typename CT = float>
 __device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue& q, uint32_t wave, uint32_t& head, uint32_t& count,
                                           bool all, uint32_t* stk, int stride, uint32_t* cnt) {
     if (count + 64u < q.cap && !(all && count > 0)) return;
@@ -159,7 +192,8 @@ __device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue
     const uint32_t lane = __lane_id();
     bool busy = false;
     RayPre r;
-    float tmax = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    float tmax = 0.0f;
+    V3<CT> C;
     int pixel = 0;
     uint32_t cur = 0, mask = 0;
     int sp = 0;
@@ -171,16 +205,10 @@ __device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue
             const uint32_t take = nidle < count ? nidle : count;
             const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
             if (!busy && rank < take) {
-                const uint32_t slot = head + rank;
-                const float4 r0 = *shq_at(q, 0, wave, slot), r1 = *shq_at(q, 1, wave, slot), r2 = *shq_at(q, 2, wave, slot);
                 TraceRay sr;
-                sr.ox = r0.x, sr.oy = r0.y, sr.oz = r0.z, sr.tmax = r0.w;
-                sr.dx = r1.x, sr.dy = r1.y, sr.dz = r1.z;
-                sr.tmin = NGI_EPS_F;
+                shq_load<CT>(q, wave, head + rank, sr, C, pixel);
                 r = make_pre(sr);
-                tmax = r0.w;
-                cr = r1.w, cg = r2.x, cb = r2.y;
-                pixel = __float_as_int(r2.z);
+                tmax = sr.tmax;
                 cur = 0;
                 sp = 0;
                 mask = sc.num_tris ? visit8<COUNT>(sc, r, 0, NGI_EPS_F, tmax, tc) : 0u;
@@ -196,7 +224,7 @@ __device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue
         if (busy && mask == 0) {
             if (sp == 0) {
                 busy = false;   // nothing in the way
-                film_add<FilmT>(a.film, pixel, V3<float>(cr, cg, cb));
+                film_add<FilmT>(a.film, pixel, C);
             } else {
                 const uint32_t e = stk[(--sp) * stride];
                 cur = e >> 8;
@@ -435,8 +463,7 @@ __device__ __forceinline__ bool vertex_begin(const RenderArgs& a, const Cam<R>& 
         const V3<R> C = nee_contribution<R>(sc, cam, prims, ps.type, ps.prim, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
         if (QUEUE && !is_zero(C)) {
             const unsigned long long m = __ballot(1);
-            shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr,
-                     V3<float>((float)C.x, (float)C.y, (float)C.z), target);
+            shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr, C, target);
             *pushed = true;
         } else if (!is_zero(C)) {
             TraceHit sh;
@@ -566,6 +593,40 @@ __device__ __forceinline__ V3<float> parked_dir(const uint32_t* pk) {
 }
 template <int S>
 __device__ __forceinline__ float parked_urr(const uint32_t* pk) { return __uint_as_float(pk[PK_URR * S]); }
+
+// FP64 shading parks the direction and the roulette number as two words
+// each: low words in rows PK_WX..PK_URR, high words in four more rows.
+template <typename R> struct ParkWords { enum { value = PK_WORDS }; };
+template <> struct ParkWords<double> { enum { value = PK_WORDS + 4 }; };
+template <int S>
+__device__ __forceinline__ void park_word2(uint32_t* pk, int lo_row, int hi_row, double x) {
+    pk[lo_row * S] = (uint32_t)__double2loint(x);
+    pk[hi_row * S] = (uint32_t)__double2hiint(x);
+}
+template <int S>
+__device__ __forceinline__ double parked_word2(const uint32_t* pk, int lo_row, int hi_row) {
+    return __hiloint2double((int)pk[hi_row * S], (int)pk[lo_row * S]);
+}
+template <int S>
+__device__ __forceinline__ void park_dir(uint32_t* pk, V3<double> wo, double urr) {
+    park_word2<S>(pk, PK_WX, PK_WORDS, wo.x);
+    park_word2<S>(pk, PK_WY, PK_WORDS + 1, wo.y);
+    park_word2<S>(pk, PK_WZ, PK_WORDS + 2, wo.z);
+    park_word2<S>(pk, PK_URR, PK_WORDS + 3, urr);
+}
+template <int S, typename R>
+__device__ __forceinline__ V3<R> parked_dir_r(const uint32_t* pk) {
+    if constexpr (std::is_same<R, double>::value)
+        return V3<R>(parked_word2<S>(pk, PK_WX, PK_WORDS), parked_word2<S>(pk, PK_WY, PK_WORDS + 1),
+                     parked_word2<S>(pk, PK_WZ, PK_WORDS + 2));
+    else
+        return parked_dir<S>(pk);
+}
+template <int S, typename R>
+__device__ __forceinline__ R parked_urr_r(const uint32_t* pk) {
+    if constexpr (std::is_same<R, double>::value) return parked_word2<S>(pk, PK_URR, PK_WORDS + 3);
+    else return parked_urr<S>(pk);
+}
 
 __device__ __forceinline__ int clamp_vertices(int m) { return (m < 0 || m > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : m; }
 

@@ -128,15 +128,30 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // 1,616 / 684, 24: 1,543 / 662 (more keep, more shading steps with fewer
 // lanes); with the drained 512-record shadow rings: keep 8: 1,753 / 709,
 // 12: 1,738 / 717, 16: 1,704 / 715.
+//
+// FP64 shading (R = double) parks its direction and roulette number in FP64
+// and queues its shadow rays with FP64 contributions (four float4s per
+// record), at NGI_PK_WAVES_FP64 waves per SIMD; on small scenes (chunked
+// path feed) it traces them in place.  Measured (M samples/s, FP64 ptdirect,
+// sphere field / nave / Cornell box): the kernel that runs every traversal
+// to its end 1,148 / 506 / 1,132; resumable with shadow rays in place 1,140
+// / 537 / 1,140; with the drained rings (rounds of 64 on the Cornell box)
+// 1,421 / 622 / 1,090.
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 8
 #endif
-template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
-__global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES) resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
-    using R = float;
-    constexpr bool QUEUE = DIRECT;
+#ifndef NGI_PK_WAVES_FP64
+#define NGI_PK_WAVES_FP64 4
+#endif
+template <typename R, bool DIRECT, bool CHUNKED>
+constexpr unsigned resume_waves() {
+    return std::is_same<R, double>::value ? NGI_PK_WAVES_FP64 : (CHUNKED && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES);
+}
+template <typename R, typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
+__global__ void __launch_bounds__(NGI_MK_BLOCK, (resume_waves<R, DIRECT, CHUNKED>())) resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    constexpr bool QUEUE = DIRECT && (std::is_same<R, float>::value || !CHUNKED);
     // LDS: extension stacks, shadow stacks (a.sc.stack_entries words per lane
-    // each), then the parked lane state (PK_WORDS per lane).
+    // each), then the parked lane state (ParkWords<R> per lane).
     extern __shared__ uint32_t stk[];
     uint32_t* ext_stk = stk + threadIdx.x;
     uint32_t* sh_stk = stk + (size_t)a.sc.stack_entries * NGI_MK_BLOCK + threadIdx.x;
@@ -156,7 +171,8 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
     for (;;) {
         // Vertices whose extension rays have finished.
         if (active && !busy)
-            active = vertex_end<R, FilmT, DIRECT>(a, prims, ps, parked_dir<NGI_MK_BLOCK>(pk), parked_urr<NGI_MK_BLOCK>(pk), parked_hit<NGI_MK_BLOCK>(pk));
+            active = vertex_end<R, FilmT, DIRECT>(a, prims, ps, parked_dir_r<NGI_MK_BLOCK, R>(pk), parked_urr_r<NGI_MK_BLOCK, R>(pk),
+                                                  parked_hit<NGI_MK_BLOCK>(pk));
         if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
             if (need) {
@@ -188,8 +204,8 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
         const bool any = __ballot(active) != 0;
         if (QUEUE) {
             qcount += (uint32_t)__popcll(__ballot(pushed));   // at most q.cap - 65 + 64 (63 + 64 chunked)
-            if (CHUNKED) shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
-            else shq_drain<FilmT, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
+            if (CHUNKED) shq_flush<FilmT, 8, COUNT, R>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
+            else shq_drain<FilmT, COUNT, R>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_MK_BLOCK, cnt);
         }
         if (!any) {
             if (feed.done()) break;
@@ -198,7 +214,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
         if (__ballot(busy)) {
             // Lanes without a ray load and store back what they parked last.
             Trav8 tv = parked_trav<NGI_MK_BLOCK>(pk);
-            const RayPre r = make_pre(extension_ray(ps, parked_dir<NGI_MK_BLOCK>(pk)));
+            const RayPre r = make_pre(extension_ray(ps, parked_dir_r<NGI_MK_BLOCK, R>(pk)));
             const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_PK_KEEP);
             trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_MK_BLOCK);
             park_trav<NGI_MK_BLOCK>(pk, tv);
@@ -232,11 +248,11 @@ static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     // lane: used while that leaves the kernel its full occupancy (on the
     // 1.8M-triangle sphere field the wide BVH is 9 levels deep: 31 KiB per
     // block, 5 blocks per CU).
-    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + PK_WORDS) * NGI_MK_BLOCK * 4;
-    const unsigned blocks_per_cu = a.path_chunk && DIRECT ? NGI_PK_WAVES_SMALL : NGI_PK_WAVES;
-    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
+    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + ParkWords<R>::value) * NGI_MK_BLOCK * 4;
+    const unsigned blocks_per_cu = a.path_chunk ? resume_waves<R, DIRECT, true>() : resume_waves<R, DIRECT, false>();
+    const bool resume = WIDTH == 8 && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
     const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4) : 0;
-    auto kern = resume ? (a.path_chunk ? resume_kernel<FilmT, DIRECT, COUNT, true> : resume_kernel<FilmT, DIRECT, COUNT, false>)
+    auto kern = resume ? (a.path_chunk ? resume_kernel<R, FilmT, DIRECT, COUNT, true> : resume_kernel<R, FilmT, DIRECT, COUNT, false>)
                        : (a.path_chunk ? persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true>
                                        : persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>);
     unsigned grid = 0;
@@ -245,14 +261,21 @@ static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     ShadowQueue q;
     q.waves = grid * (NGI_MK_BLOCK / 64);
     q.rec = nullptr;
-    if (DIRECT && std::is_same<R, float>::value) {
+    if (DIRECT && (std::is_same<R, float>::value || (resume && !a.path_chunk))) {
         // Longer rings for the refilling drains; a caller-owned buffer gets
-        // the longest ring it holds.
+        // the longest ring it holds, and as many blocks as it has rings for
+        // (the persistent grid takes every path with fewer blocks too).
+        const size_t rec = (size_t)ShqWords<R>::value * sizeof(float4);
         if (resume && !a.path_chunk) {
             q.cap = NGI_SHQ_LONG;
-            while (mem.fixed && q.cap > NGI_SHQ && (size_t)3 * q.waves * q.cap * sizeof(float4) > *mem.bytes) q.cap >>= 1;
+            while (mem.fixed && q.cap > NGI_SHQ && rec * q.waves * q.cap > *mem.bytes) q.cap >>= 1;
         }
-        if ((s = mem.reserve((size_t)3 * q.waves * q.cap * sizeof(float4)))) return s;
+        if (mem.fixed && rec * q.waves * q.cap > *mem.bytes) {
+            grid = (unsigned)std::min<size_t>(grid, *mem.bytes / (rec * q.cap * (NGI_MK_BLOCK / 64)));
+            if (!grid) return fail(NGI_ERR_INVALID_ARG, "the fixed buffer of %zu bytes holds no block's shadow rings", *mem.bytes);
+            q.waves = grid * (NGI_MK_BLOCK / 64);
+        }
+        if ((s = mem.reserve(rec * q.waves * q.cap))) return s;
         q.rec = reinterpret_cast<float4*>(*mem.ptr);
     }
     kern<<<grid, NGI_MK_BLOCK, lds, st>>>(a, next, q);

@@ -247,6 +247,24 @@ def test_persistent_engine_matches_megakernel(renderer):
     _crn_close(g, m, 2e-4)
 
 
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
+def test_fp64_persistent_engine_matches_megakernel(renderer):
+    """FP64 shading on the persistent engine (resumable traversals, NEE shadow
+    rays traced in place) against the FP64 megakernel: the same paths with the
+    same FP64 arithmetic, so only the order of the film's FP64 additions
+    differs."""
+    s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=4)
+    s.build(ngi.BVH_8)
+    out = {}
+    for e in (ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT):
+        out[e] = ngi.render(s, _params(renderer, 40, 30, 8, 12, rr_depth=3, engine=e, precision=ngi.PRECISION_FP64_ALL))
+    (g, st), (m, st2) = out[ngi.ENGINE_PERSISTENT], out[ngi.ENGINE_MEGAKERNEL]
+    assert st["samples"] == st2["samples"] == 40 * 30 * 8
+    for k in ("extension_rays", "shadow_rays"):
+        assert abs(st[k] - st2[k]) <= 1e-4 * st2[k], k
+    _crn_close(g, m, 1e-9, frac=0.999, mean_rtol=1e-4)
+
+
 def test_deep_wide_bvh_takes_full_length_traversals():
     """A wide BVH 10 levels deep (5.8M triangles) needs more LDS per lane
     for resumable traversals (two stacks of 11 entries and the parked state)
