@@ -50,10 +50,12 @@ def test_cornell_common_random_numbers(engine, renderer, width):
 # FP64 shading still starts every bounce from an FP32 hit distance (the
 # traversal is FP32, as Embree's is in the reference), so it agrees with the
 # oracle to the same per-pixel tolerance as FP32 shading.
+@pytest.mark.parametrize("engine", [ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT])
+@pytest.mark.parametrize("renderer", ["pt", "ptdirect"])
 @pytest.mark.parametrize("precision,tol", [(ngi.PRECISION_FP64_ALL, 2e-4), (ngi.PRECISION_FP32_ALL, 3e-4)])
-def test_precision_variants(precision, tol):
+def test_precision_variants(precision, tol, renderer, engine):
     s = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
-    p = _params("ptdirect", 32, 32, 8, 9, rr_depth=2, precision=precision)
+    p = _params(renderer, 32, 32, 8, 9, rr_depth=2, precision=precision, engine=engine)
     g, _ = ngi.render(s, p)
     _crn_close(g, oracle.render(s, p), tol)
 
