@@ -172,10 +172,11 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
-    // Resumable traversals while their LDS leaves the kernel its occupancy.
+    // Resumable traversals while their LDS leaves room for
+    // NGI_RESUME_MIN_BLOCKS blocks per CU.
     const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + PK_WORDS) * NGI_LT_BLOCK * 4;
     const unsigned blocks_per_cu = a.path_chunk && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES;
-    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
+    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * resume_min_blocks(blocks_per_cu) <= NGI_CU_LDS_BYTES;
     const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4) : 0;
     auto kern = resume ? (a.path_chunk ? light_resume_kernel<FilmT, DIRECT, COUNT, true> : light_resume_kernel<FilmT, DIRECT, COUNT, false>)
                        : (a.path_chunk ? light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>);

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <float.h>
+#include <stdlib.h>
 
 #include <type_traits>
 
@@ -551,6 +552,22 @@ __device__ __forceinline__ bool path_vertex(const RenderArgs& a, const Cam<R>& c
 // lane, S words apart.
 enum { PK_CUR = 0, PK_MASK, PK_SP, PK_T, PK_U, PK_V, PK_LEAF, PK_WX, PK_WY, PK_WZ, PK_URR, PK_WORDS };
 #define NGI_CU_LDS_BYTES (160u * 1024u)   // LDS per CU on gfx950
+// Fewest blocks per CU (LDS-bound) a resumable kernel may run at on deep
+// trees before the launchers fall back to the kernels that run every
+// traversal to its end: 4, or the environment's NGI_RESUME_MIN_BLOCKS (1 to
+// the kernel's register budget; read at every launch).  Measured on the
+// 5.8M-triangle sphere field (wide BVH 10 levels deep, 33 KiB per block):
+// ptdirect 1,229 -> 1,377 M samples/s, pt 1,970 -> 2,162 M, ltdirect
+// 1,393 -> 1,723 M at 4 blocks against the fallback at 5.
+#define NGI_RESUME_MIN_BLOCKS 4u
+inline unsigned resume_min_blocks(unsigned budget) {
+    unsigned m = NGI_RESUME_MIN_BLOCKS;
+    if (const char* e = getenv("NGI_RESUME_MIN_BLOCKS")) {
+        const long v = strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 64) m = (unsigned)v;
+    }
+    return budget < m ? budget : m;
+}
 
 template <int S>
 __device__ __forceinline__ void park_trav(uint32_t* pk, const Trav8& t) {

@@ -245,12 +245,12 @@ static void launch_mk3(const RenderArgs& a, bool direct, int width, hipStream_t 
 template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_pk(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     // Resumable traversals need a second LDS stack and the parked state per
-    // lane: used while that leaves the kernel its full occupancy (on the
-    // 1.8M-triangle sphere field the wide BVH is 9 levels deep: 31 KiB per
-    // block, 5 blocks per CU).
+    // lane: used while that leaves room for NGI_RESUME_MIN_BLOCKS blocks per
+    // CU (on the 1.8M-triangle sphere field the wide BVH is 9 levels deep:
+    // 31 KiB per block, the full 5 blocks per CU; deeper trees run at 4).
     const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + ParkWords<R>::value) * NGI_MK_BLOCK * 4;
     const unsigned blocks_per_cu = a.path_chunk ? resume_waves<R, DIRECT, true>() : resume_waves<R, DIRECT, false>();
-    const bool resume = WIDTH == 8 && resume_lds * blocks_per_cu <= NGI_CU_LDS_BYTES;
+    const bool resume = WIDTH == 8 && resume_lds * resume_min_blocks(blocks_per_cu) <= NGI_CU_LDS_BYTES;
     const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_MK_BLOCK * 4) : 0;
     auto kern = resume ? (a.path_chunk ? resume_kernel<R, FilmT, DIRECT, COUNT, true> : resume_kernel<R, FilmT, DIRECT, COUNT, false>)
                        : (a.path_chunk ? persistent_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true>

@@ -267,22 +267,29 @@ def test_fp64_persistent_engine_matches_megakernel(renderer):
     _crn_close(g, m, 1e-9, frac=0.999, mean_rtol=1e-4)
 
 
-def test_deep_wide_bvh_takes_full_length_traversals():
-    """A wide BVH 10 levels deep (5.8M triangles) needs more LDS per lane
-    for resumable traversals (two stacks of 11 entries and the parked state)
-    than 5 wavefronts per SIMD leave, so FP32 renders fall back to the kernel
-    that runs every traversal to its end; it must agree with the megakernel
-    path by path."""
+def test_deep_wide_bvh_resumable_at_four_blocks_and_fallback(monkeypatch):
+    """A wide BVH 10 levels deep (5.8M triangles) needs 33 KiB of LDS per
+    block for resumable traversals (two stacks of 11 entries and the parked
+    state per lane): 4 blocks per CU fit, 5 do not.  By default the
+    resumable kernels then run at 4 waves/SIMD; with NGI_RESUME_MIN_BLOCKS=5
+    the launchers fall back to the kernels that run every traversal to its
+    end.  ptdirect must agree with the megakernel path by path either way,
+    and ltdirect (persistent only) between the two kernels."""
     s = ngi.Scene.generate(ngi.GEN_SPHERE_FIELD, 1, 1.0, detail=9)
     s.build(ngi.BVH_8)
-    assert s.bvh_info()["max_depth"] >= 10
-    out = {}
-    for e in (ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT):
-        out[e] = ngi.render(s, _params("ptdirect", 48, 32, 8, 12, rr_depth=3, engine=e))
-    (g, st), (m, st2) = out[ngi.ENGINE_PERSISTENT], out[ngi.ENGINE_MEGAKERNEL]
-    assert st["samples"] == st2["samples"] == 48 * 32 * 8
-    assert abs(st["extension_rays"] - st2["extension_rays"]) <= 1e-3 * st2["extension_rays"]
-    _crn_close(g, m, 2e-4)
+    assert 10 <= s.bvh_info()["max_depth"] <= 13
+    m, st_m = ngi.render(s, _params("ptdirect", 48, 32, 8, 12, rr_depth=3, engine=ngi.ENGINE_MEGAKERNEL))
+    lt = {}
+    for min_blocks in ("4", "5"):
+        monkeypatch.setenv("NGI_RESUME_MIN_BLOCKS", min_blocks)
+        g, st = ngi.render(s, _params("ptdirect", 48, 32, 8, 12, rr_depth=3, engine=ngi.ENGINE_PERSISTENT))
+        assert st["samples"] == st_m["samples"] == 48 * 32 * 8
+        assert abs(st["extension_rays"] - st_m["extension_rays"]) <= 1e-3 * st_m["extension_rays"], min_blocks
+        _crn_close(g, m, 2e-4)
+        lt[min_blocks] = ngi.render(s, _params("ltdirect", 48, 32, 8, 12, rr_depth=3, engine=ngi.ENGINE_PERSISTENT))
+    (a, sa), (b, sb) = lt["4"], lt["5"]
+    assert sa["samples"] == sb["samples"] and abs(sa["extension_rays"] - sb["extension_rays"]) <= 1e-3 * sb["extension_rays"]
+    _crn_close(a, b, 2e-4)
 
 
 @pytest.mark.parametrize("width", [ngi.BVH_2, ngi.BVH_8])
