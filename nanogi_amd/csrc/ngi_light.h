@@ -125,8 +125,8 @@ __device__ __forceinline__ bool light_begin(const RenderArgs& a, const Cam<R>& c
                     sr.tmax = (float)len * (1.0f - NGI_EPS_F);   // Scene::Visible (rt.hpp:2251-2261)
                     if (QUEUE) {
                         const unsigned long long m = __ballot(1);
-                        shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr,
-                                 V3<float>((float)C.x, (float)C.y, (float)C.z), pixel_index(rx, ry, a.w, a.h));
+                        shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr, C,
+                                 pixel_index(rx, ry, a.w, a.h));
                         *pushed = true;
                     } else {
                         TraceHit sh;

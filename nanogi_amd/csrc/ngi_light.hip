@@ -83,14 +83,23 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
 // through the shadow rings: drained with refills from rings of
 // NGI_SHQ_LONG records on large scenes, in rounds of 64 on small ones.
 // Measured (ltdirect, M samples/s, sphere field / nave): keep 4: 1,952 / 716,
-// 8: 2,061 / 742, 16: 2,098 / 743, 24: 1,924 / 660.
+// 8: 2,061 / 742, 16: 2,098 / 743, 24: 1,924 / 660.  FP64 shading (R =
+// double) parks its direction and roulette number in FP64, queues its
+// sensor connections with FP64 contributions on large scenes and makes them
+// in place on small ones, at NGI_LT_WAVES_FP64 waves per SIMD.
 #ifndef NGI_LT_KEEP
 #define NGI_LT_KEEP 16
 #endif
-template <typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
-__global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES) light_resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
-    using R = float;
-    constexpr bool QUEUE = DIRECT;
+#ifndef NGI_LT_WAVES_FP64
+#define NGI_LT_WAVES_FP64 4
+#endif
+template <typename R, bool DIRECT, bool CHUNKED>
+constexpr unsigned light_resume_waves() {
+    return std::is_same<R, double>::value ? NGI_LT_WAVES_FP64 : (CHUNKED && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES);
+}
+template <typename R, typename FilmT, bool DIRECT, bool COUNT, bool CHUNKED>
+__global__ void __launch_bounds__(NGI_LT_BLOCK, (light_resume_waves<R, DIRECT, CHUNKED>())) light_resume_kernel(RenderArgs a, unsigned long long* next, ShadowQueue q) {
+    constexpr bool QUEUE = DIRECT && (std::is_same<R, float>::value || !CHUNKED);
     // LDS: extension stacks, shadow stacks (a.sc.stack_entries words per
     // lane each), then the parked lane state.
     extern __shared__ uint32_t stk[];
@@ -111,7 +120,8 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
     bool busy = false;     // ... whose extension ray is being traced
     for (;;) {
         // Vertices whose extension rays have finished.
-        if (active && !busy) active = light_end<R>(a, prims, ls, parked_dir<NGI_LT_BLOCK>(pk), parked_urr<NGI_LT_BLOCK>(pk), parked_hit<NGI_LT_BLOCK>(pk));
+        if (active && !busy)
+            active = light_end<R>(a, prims, ls, parked_dir_r<NGI_LT_BLOCK, R>(pk), parked_urr_r<NGI_LT_BLOCK, R>(pk), parked_hit<NGI_LT_BLOCK>(pk));
         if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
             if (need) {
@@ -142,8 +152,8 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
         const bool any = __ballot(active) != 0;
         if (QUEUE) {
             qcount += (uint32_t)__popcll(__ballot(pushed));   // at most q.cap - 65 + 64 (63 + 64 chunked)
-            if (CHUNKED) shq_flush<FilmT, 8, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
-            else shq_drain<FilmT, COUNT>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
+            if (CHUNKED) shq_flush<FilmT, 8, COUNT, R>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
+            else shq_drain<FilmT, COUNT, R>(a, q, wave, qhead, qcount, !any && feed.done(), sh_stk, NGI_LT_BLOCK, cnt);
         }
         if (!any) {
             if (feed.done()) break;
@@ -152,8 +162,9 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
         if (__ballot(busy)) {
             // Lanes without a ray load and store back what they parked last.
             Trav8 tv = parked_trav<NGI_LT_BLOCK>(pk);
-            const V3<R> d = parked_dir<NGI_LT_BLOCK>(pk);
-            const RayPre r = make_pre(TraceRay{ls.s.p.x, ls.s.p.y, ls.s.p.z, d.x, d.y, d.z, NGI_EPS_F, FLT_MAX});
+            const V3<R> d = parked_dir_r<NGI_LT_BLOCK, R>(pk);
+            const RayPre r = make_pre(TraceRay{(float)ls.s.p.x, (float)ls.s.p.y, (float)ls.s.p.z, (float)d.x, (float)d.y, (float)d.z, NGI_EPS_F,
+                                               FLT_MAX});
             const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_LT_KEEP);
             trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_LT_BLOCK);
             park_trav<NGI_LT_BLOCK>(pk, tv);
@@ -174,11 +185,11 @@ template <typename R, typename FilmT, bool DIRECT, int WIDTH, bool COUNT>
 static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem, Lanes ln, hipStream_t st) {
     // Resumable traversals while their LDS leaves room for
     // NGI_RESUME_MIN_BLOCKS blocks per CU.
-    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + PK_WORDS) * NGI_LT_BLOCK * 4;
-    const unsigned blocks_per_cu = a.path_chunk && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES;
-    const bool resume = WIDTH == 8 && std::is_same<R, float>::value && resume_lds * resume_min_blocks(blocks_per_cu) <= NGI_CU_LDS_BYTES;
+    const size_t resume_lds = ((size_t)2 * a.sc.stack_entries + ParkWords<R>::value) * NGI_LT_BLOCK * 4;
+    const unsigned blocks_per_cu = a.path_chunk ? light_resume_waves<R, DIRECT, true>() : light_resume_waves<R, DIRECT, false>();
+    const bool resume = WIDTH == 8 && resume_lds * resume_min_blocks(blocks_per_cu) <= NGI_CU_LDS_BYTES;
     const size_t lds = WIDTH == 8 ? (resume ? resume_lds : (size_t)a.sc.stack_entries * NGI_LT_BLOCK * 4) : 0;
-    auto kern = resume ? (a.path_chunk ? light_resume_kernel<FilmT, DIRECT, COUNT, true> : light_resume_kernel<FilmT, DIRECT, COUNT, false>)
+    auto kern = resume ? (a.path_chunk ? light_resume_kernel<R, FilmT, DIRECT, COUNT, true> : light_resume_kernel<R, FilmT, DIRECT, COUNT, false>)
                        : (a.path_chunk ? light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, true> : light_kernel<R, FilmT, DIRECT, WIDTH, COUNT, false>);
     unsigned grid = 0;
     int s = persistent_grid(reinterpret_cast<const void*>(kern), NGI_LT_BLOCK, lds, a.path_count, ln, &grid);
@@ -186,14 +197,20 @@ static int launch_lt(const RenderArgs& a, unsigned long long* next, LaneMem mem,
     ShadowQueue q;
     q.waves = grid * (NGI_LT_BLOCK / 64);
     q.rec = nullptr;
-    if (DIRECT && std::is_same<R, float>::value) {
+    if (DIRECT && (std::is_same<R, float>::value || (resume && !a.path_chunk))) {
         // Longer rings for the refilling drains; a caller-owned buffer gets
-        // the longest ring it holds.
+        // the longest ring it holds, and as many blocks as it has rings for.
+        const size_t rec = (size_t)ShqWords<R>::value * sizeof(float4);
         if (resume && !a.path_chunk) {
             q.cap = NGI_SHQ_LONG;
-            while (mem.fixed && q.cap > NGI_SHQ && (size_t)3 * q.waves * q.cap * sizeof(float4) > *mem.bytes) q.cap >>= 1;
+            while (mem.fixed && q.cap > NGI_SHQ && rec * q.waves * q.cap > *mem.bytes) q.cap >>= 1;
         }
-        if ((s = mem.reserve((size_t)3 * q.waves * q.cap * sizeof(float4)))) return s;
+        if (mem.fixed && rec * q.waves * q.cap > *mem.bytes) {
+            grid = (unsigned)std::min<size_t>(grid, *mem.bytes / (rec * q.cap * (NGI_LT_BLOCK / 64)));
+            if (!grid) return fail(NGI_ERR_INVALID_ARG, "the fixed buffer of %zu bytes holds no block's shadow rings", *mem.bytes);
+            q.waves = grid * (NGI_LT_BLOCK / 64);
+        }
+        if ((s = mem.reserve(rec * q.waves * q.cap))) return s;
         q.rec = reinterpret_cast<float4*>(*mem.ptr);
     }
     kern<<<grid, NGI_LT_BLOCK, lds, st>>>(a, next, q);

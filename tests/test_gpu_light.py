@@ -105,15 +105,18 @@ def test_ltdirect_through_glass(spec):
     _crn_close(g, o, 5e-4, frac=0.95, mean_rtol=5e-3)
 
 
+@pytest.mark.parametrize("precision", [ngi.PRECISION_FP32_FP64ACC, ngi.PRECISION_FP64_ALL])
 @pytest.mark.parametrize("kind,detail,pix_rtol", [(ngi.GEN_NAVE, 0, 5e-3), (ngi.GEN_SPHERE_FIELD, 3, 5e-4)])
-def test_ltdirect_nave_and_glossy_sphere_field(kind, detail, pix_rtol):
-    """Light paths of up to 10 vertices over thousands of triangles.  Each
-    light path splats into up to nine pixels, so the few paths whose FP32
-    rays take other geometry than the FP64 ones (57 of 1.18 M extension rays
-    on the nave) touch many pixels by about 1/500 each: on the nave the
-    per-pixel bar is 5e-3; the image means agree to 2e-5 there."""
+def test_ltdirect_nave_and_glossy_sphere_field(kind, detail, pix_rtol, precision):
+    """Light paths of up to 10 vertices over thousands of triangles, with
+    FP32 and FP64 shading (both scenes are large enough for the sensor
+    connections to go through the shadow rings).  Each light path splats
+    into up to nine pixels, so the few paths whose FP32 rays take other
+    geometry than the FP64 ones (57 of 1.18 M extension rays on the nave)
+    touch many pixels by about 1/500 each: on the nave the per-pixel bar is
+    5e-3; the image means agree to 2e-5 there."""
     s = ngi.Scene.generate(kind, 1, 1.0, detail)
-    p = _p("ltdirect", 24, 24, 300000, 10, seed=5, rr_depth=3)
+    p = _p("ltdirect", 24, 24, 300000, 10, seed=5, rr_depth=3, precision=precision)
     g, st = ngi.render(s, p)
     o = oracle.render(s, p)
     assert np.abs(o).max() > 0 and st["shadow_rays"] > 0

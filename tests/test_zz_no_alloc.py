@@ -86,6 +86,7 @@ CASES = [("pt", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 1e-9),
          ("ptdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 1e-9),
          ("lt", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 1e-9),
          ("ltdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_FP64ACC, 1e-9),
+         ("ltdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL, 1e-9),
          ("ptdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP64_ALL, 1e-9),
          ("ptdirect", ngi.ENGINE_PERSISTENT, ngi.PRECISION_FP32_ALL, 1e-5),
          ("pt", ngi.ENGINE_MEGAKERNEL, ngi.PRECISION_FP32_FP64ACC, 1e-9),
