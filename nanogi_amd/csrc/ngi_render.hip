@@ -136,7 +136,9 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // sphere field / nave / Cornell box): the kernel that runs every traversal
 // to its end 1,148 / 506 / 1,132; resumable with shadow rays in place 1,140
 // / 537 / 1,140; with the drained rings (rounds of 64 on the Cornell box)
-// 1,421 / 622 / 1,090.
+// 1,421 / 622 / 1,090.  Register budgets (sphere field / nave, ptdirect):
+// 4 waves/SIMD (128 VGPRs) 1,422 / 623, 3 waves 1,300 / 541, 5 waves 1,194
+// / 586.
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 8
 #endif
