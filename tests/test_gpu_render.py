@@ -314,6 +314,45 @@ def test_scene_without_triangles(width):
             assert (np.count_nonzero(o.sum(axis=2)) == 1) == (r not in ("pt", "lt")), r
 
 
+def test_fp64_shading_edge_cases():
+    """FP64 shading (resumable persistent kernels with FP64 parked state and
+    FP64-contribution queues) on the edge cases: no triangles at all, paths of
+    0 to 3 vertices, and images smaller than a wavefront or ragged, against
+    the oracle (pt / ptdirect also on the FP64 megakernel)."""
+    P = ngi.PRECISION_FP64_ALL
+    B = scenes.SceneBuilder()
+    B.prim(type=ngi.PRIM_L, light_type=ngi.LIGHT_POINT, Le=(2, 1, 1), light_position=(0.1, 0.2, -2.0))
+    B.camera((0, 0, 0), (0, 0, -1))
+    empty = ngi.Scene.create(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64), np.zeros(0, dtype=np.int64),
+                             np.array(B.prims), normals=np.zeros((0, 3)))
+    empty.build(ngi.BVH_8)
+    corn = ngi.Scene.generate(ngi.GEN_CORNELL, 1, 1.0)
+    corn.build(ngi.BVH_8)
+    for r in ("pt", "ptdirect", "lt", "ltdirect"):
+        engines = [ngi.ENGINE_MEGAKERNEL, ngi.ENGINE_PERSISTENT] if r in ("pt", "ptdirect") else [ngi.ENGINE_PERSISTENT]
+        for eng in engines:
+            p = ngi.default_params(renderer=r, width=16, height=16, sampling=ngi.SAMPLING_PER_PIXEL, spp=4,
+                                   max_num_vertices=4, seed=3, engine=eng, precision=P)
+            g, _ = ngi.render(empty, p)
+            np.testing.assert_allclose(g, oracle.render(empty, p), rtol=1e-5, atol=1e-9, err_msg=f"{r} {eng}")
+            for m in (0, 1, 2, 3):
+                p = ngi.default_params(renderer=r, width=16, height=16, sampling=ngi.SAMPLING_PER_PIXEL, spp=8,
+                                       max_num_vertices=m, seed=2, engine=eng, precision=P)
+                g, _ = ngi.render(corn, p)
+                o = oracle.render(corn, p)
+                if m <= 1 or not o.any():
+                    assert not g.any(), (r, m, eng)
+                else:
+                    _crn_close(g, o, 2e-4, frac=0.97)
+            for w, h in ((1, 1), (7, 3), (65, 17)):
+                p = ngi.default_params(renderer=r, width=w, height=h, sampling=ngi.SAMPLING_PER_PIXEL, spp=3,
+                                       max_num_vertices=5, seed=4, engine=eng, precision=P)
+                g, st = ngi.render(corn, p)
+                assert st["samples"] == w * h * 3
+                if r in ("pt", "ptdirect"):
+                    _crn_close(g, oracle.render(corn, p), 2e-4, frac=0.97)
+
+
 @pytest.mark.parametrize("renderer", ["pt", "ptdirect", "lt", "ltdirect", "bdpt"])
 def test_shortest_path_limits(renderer):
     """-m 0 and -m 1 render nothing; -m 2 and -m 3 are the direct-view and
