@@ -244,17 +244,18 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
                 TraceCounters tc{0, 0};
                 alive = trace<WIDTH, false, false>(sc, er, h, tc, my_stk, NGI_ADJ_BLOCK);
                 if (alive) {
-                    int hp;
-                    const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp);
-                    const DevPrim<float>& hm = prims[hp];
-                    if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA &&
-                        dot(ns.n, -wo) > 0.0f)
-                        adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                    int hp, htype;
+                    const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp, htype);
+                    if (!DIRECT && (htype & NGI_PRIM_L)) {
+                        const DevPrim<float>& hm = prims[hp];
+                        if (hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA && dot(ns.n, -wo) > 0.0f)
+                            adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                    }
                     if (a.rr_depth >= 0 && nv >= a.rr_depth) {
                         if (u24f(rx.w) > a.rr_prob_f) alive = false;
                         else T = T / a.rr_prob_f;
                     }
-                    type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                    type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
                     if (type == 0) alive = false;
                     s = ns;
                     prim = hp;
@@ -325,16 +326,18 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_resume_k
             bool alive = h.leaf >= 0;
             if (alive) {
                 const f3 wo = parked_dir<NGI_ADJ_BLOCK>(pk);
-                int hp;
-                const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp);
-                const DevPrim<float>& hm = prims[hp];
-                if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA && dot(ns.n, -wo) > 0.0f)
-                    adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                int hp, htype;
+                const Surf<float> ns = hit_surface<float>(sc, s.p, wo, h, hp, htype);
+                if (!DIRECT && (htype & NGI_PRIM_L)) {
+                    const DevPrim<float>& hm = prims[hp];
+                    if (hm.light >= 0 && lights[hm.light].type == NGI_LIGHT_AREA && dot(ns.n, -wo) > 0.0f)
+                        adj_contribute<ZEROS>(g, prims, sL, nlds, q, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]), hp);
+                }
                 if (a.rr_depth >= 0 && nv >= a.rr_depth) {
                     if (parked_urr<NGI_ADJ_BLOCK>(pk) > a.rr_prob_f) alive = false;
                     else T = T / a.rr_prob_f;
                 }
-                type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
                 if (type == 0) alive = false;
                 s = ns;
                 prim = hp;

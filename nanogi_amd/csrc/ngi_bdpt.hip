@@ -610,12 +610,12 @@ __device__ __forceinline__ bool bd_direction(const RenderArgs& a, const BdptCtx&
 // The vertex where the ray from pv along wo hit (h.leaf >= 0); alpha is the
 // subpath throughput up to it.
 __device__ __forceinline__ BVert bd_vertex(const BdptCtx& c, const BVert& pv, f3 wo, const TraceHit& h, f3 alpha) {
-    int hp;
-    const Surf<float> ns = hit_surface<float>(*c.sc, pv.p, wo, h, hp);
+    int hp, htype;
+    const Surf<float> ns = hit_surface<float>(*c.sc, pv.p, wo, h, hp, htype);
     BVert v;
     v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
     v.prim = hp;
-    v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    v.type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
     v.alpha = alpha;
     return v;
 }
@@ -833,12 +833,12 @@ __global__ void __launch_bounds__(NGI_BD_BLOCK, NGI_BD_WAVES) bdpt_kernel(Render
                     cnt[CNT_EXT_RAYS]++;
                     if (COUNT) cnt[CNT_EXT_NODES] += tc.nodes, cnt[CNT_EXT_TRIS] += tc.tris;
                     if (!hit) break;
-                    int hp;
-                    const Surf<float> ns = hit_surface<float>(sc, pv.p, wo, h, hp);
+                    int hp, htype;
+                    const Surf<float> ns = hit_surface<float>(sc, pv.p, wo, h, hp, htype);
                     BVert v;
                     v.p = ns.p, v.gn = ns.gn, v.n = ns.n, v.deg = false;
                     v.prim = hp;
-                    v.type = c.prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+                    v.type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
                     v.alpha = pv.alpha * wgt;
                     bstore(mem, side, step, gl, v);
                     nv = step + 1;

@@ -3,7 +3,7 @@
 // HBM layout (all arrays 16-byte aligned, leaf order = BVH leaf order):
 //   nodes      : Bvh2Node[] (64 B) or Bvh8Node[] (128 B)
 //   tris       : 3 x float4 per leaf triangle: (v0, orig id), (v1, 0), (v2, 0)
-//   shade      : 3 x float4 per leaf triangle: (n0, prim id), (n1, 0), (n2, 0)
+//   shade      : 3 x float4 per leaf triangle: (n0, prim id), (n1, prim type bits), (n2, 0)
 //   prims_f/_d : DevPrim<float>/<double> per primitive
 //   lights_f/_d: DevLight<T> per light primitive (Scene::LightPrimitiveIndices)
 //   light_tris : 3 x float4 per area-light triangle, per light contiguous

@@ -174,6 +174,9 @@ int device_prepare(HostScene& s) {
                 uint32_t bits = t, pbits = s.tri_prim[t];
                 memcpy(&w, &bits, 4);
                 memcpy(&wn, &pbits, 4);
+            } else if (k == 1) {
+                const uint32_t tbits = (uint32_t)s.prims[s.tri_prim[t]].type;   // fixed by the scene (set_params keeps it)
+                memcpy(&wn, &tbits, 4);
             }
             tris[3 * j + k] = make_float4((float)s.positions[3 * (size_t)vi], (float)s.positions[3 * (size_t)vi + 1],
                                           (float)s.positions[3 * (size_t)vi + 2], w);

@@ -171,14 +171,14 @@ template <typename R>
 __device__ __forceinline__ bool light_end(const RenderArgs& a, const DevPrim<R>* prims, LightState<R>& ls, V3<R> wo, R urr, const TraceHit& h) {
     const DevScene& sc = a.sc;
     if (h.leaf < 0) return false;
-    int hp;
-    const Surf<R> ns = hit_surface<R>(sc, ls.s.p, wo, h, hp);
+    int hp, htype;
+    const Surf<R> ns = hit_surface<R>(sc, ls.s.p, wo, h, hp, htype);
     if (a.rr_depth >= 0 && ls.nv >= a.rr_depth) {   // src/nanogi.cpp:910-921
         const R q = rr_prob<R>(a);
         if (urr > q) return false;
         ls.T = ls.T / q;
     }
-    ls.type = prims[hp].type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    ls.type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
     if (ls.type == 0) return false;   // [L]-only surfaces: the next EvaluateDirection is zero
     ls.s = ns;
     ls.prim = hp;

@@ -256,13 +256,17 @@ __device__ __forceinline__ void shq_drain(const RenderArgs& a, const ShadowQueue
 // geometric normal from the winding (2206), shading normal interpolated
 // from vertex normals with weights (1 - u - v, u, v) computed in FP32 as
 // Embree's u, v are (2209-2218), NaN -> geometric normal.
+// The owning primitive's type bits come with the shading record, so a
+// vertex needs the primitive's record only for what its BSDF or emission
+// reads.
 template <typename R>
-__device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R> d, const TraceHit& h, int& prim) {
+__device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R> d, const TraceHit& h, int& prim, int& type) {
     const float4* tv = sc.tris + 3 * (size_t)h.leaf;
     const float4* sh = sc.shade + 3 * (size_t)h.leaf;
     const float4 v0 = tv[0], v1 = tv[1], v2 = tv[2];
     const float4 n0 = sh[0], n1 = sh[1], n2 = sh[2];
     prim = __float_as_int(n0.w);
+    type = __float_as_int(n1.w);
     Surf<R> s;
     s.p = o + d * (R)h.t;
     const V3<R> p1((R)v0.x, (R)v0.y, (R)v0.z), p2((R)v1.x, (R)v1.y, (R)v1.z), p3((R)v2.x, (R)v2.y, (R)v2.z);
@@ -275,6 +279,11 @@ __device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R
     s.n = sn;
     s.degenerate = false;
     return s;
+}
+template <typename R>
+__device__ __forceinline__ Surf<R> hit_surface(const DevScene& sc, V3<R> o, V3<R> d, const TraceHit& h, int& prim) {
+    int type;
+    return hit_surface<R>(sc, o, d, h, prim, type);
 }
 
 // Next-event estimation at one vertex (src/nanogi.cpp:656-710).  Returns the
@@ -505,13 +514,13 @@ __device__ __forceinline__ bool vertex_end(const RenderArgs& a, const DevPrim<R>
                                            const TraceHit& h) {
     const DevScene& sc = a.sc;
     if (h.leaf < 0) return false;
-    int hp;
-    const Surf<R> ns = hit_surface<R>(sc, ps.s.p, wo, h, hp);
-    const DevPrim<R>& hm = prims[hp];
-    if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
+    int hp, htype;
+    const Surf<R> ns = hit_surface<R>(sc, ps.s.p, wo, h, hp, htype);
+    if (!DIRECT && (htype & NGI_PRIM_L)) {
         // Emission on hit (src/nanogi.cpp:568-575): area lights only.
-        const DevLight<R>& L = lights_of<R>(sc)[hm.light];
-        if (L.type == NGI_LIGHT_AREA && dot(ns.n, -wo) > R(0))
+        const DevPrim<R>& hm = prims[hp];
+        const DevLight<R>* L = hm.light >= 0 ? &lights_of<R>(sc)[hm.light] : nullptr;
+        if (L && L->type == NGI_LIGHT_AREA && dot(ns.n, -wo) > R(0))
             film_add<FilmT>(a.film, ps.pixel, ps.T * V3<R>(hm.Le[0], hm.Le[1], hm.Le[2]));
     }
     if (a.rr_depth >= 0 && ps.nv >= a.rr_depth) {   // src/nanogi.cpp:583-591
@@ -519,7 +528,7 @@ __device__ __forceinline__ bool vertex_end(const RenderArgs& a, const DevPrim<R>
         if (urr > q) return false;
         ps.T = ps.T / q;
     }
-    ps.type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+    ps.type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
     if (ps.type == 0) return false;   // [L]-only primitives absorb (rt.hpp:1146-1147)
     ps.s = ns;
     ps.prim = hp;

@@ -176,12 +176,15 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
             c0 = s1.x, c1 = s1.y, nv = (int)s1.z;
             TraceHit h;
             h.t = h4.x, h.u = h4.y, h.v = h4.z, h.leaf = leaf;
-            ns = hit_surface<float>(sc, o, d, h, hp);
+            int htype;
+            ns = hit_surface<float>(sc, o, d, h, hp, htype);
             const DevPrim<float>* prims = prims_of<float>(sc);
-            const DevPrim<float>& hm = prims[hp];
-            if (!DIRECT && (hm.type & NGI_PRIM_L) && hm.light >= 0) {
-                const DevLight<float>& L = lights_of<float>(sc)[hm.light];
-                if (L.type == NGI_LIGHT_AREA && dot(ns.n, -d) > 0.0f) film_add<FilmT>(a.film, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]));
+            if (!DIRECT && (htype & NGI_PRIM_L)) {
+                const DevPrim<float>& hm = prims[hp];
+                if (hm.light >= 0) {
+                    const DevLight<float>& L = lights_of<float>(sc)[hm.light];
+                    if (L.type == NGI_LIGHT_AREA && dot(ns.n, -d) > 0.0f) film_add<FilmT>(a.film, pixel, T * f3(hm.Le[0], hm.Le[1], hm.Le[2]));
+                }
             }
             const U4 rx = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
             alive = true;
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
                 if (u24f(rx.w) > a.rr_prob_f) alive = false;
                 else T = T / a.rr_prob_f;
             }
-            type = hm.type & ~(NGI_PRIM_L | NGI_PRIM_E);
+            type = htype & ~(NGI_PRIM_L | NGI_PRIM_E);
             if (type == 0) alive = false;
             nv++;
             const int max_nv = (a.max_nv < 0 || a.max_nv > NGI_MAX_VERTICES) ? NGI_MAX_VERTICES : a.max_nv;
@@ -204,7 +207,7 @@ __global__ void __launch_bounds__(NGI_WF_BLOCK) wf_shade(RenderArgs a, const flo
             if (alive) {
                 const U4 ry = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_EXT, a.seed, a.stream_id);
                 f3 wgt;
-                if (bsdf_sample<float>(hm, type, ns, wi, u24f(ry.x), u24f(ry.y), u24f(ry.z), wo, wgt)) T = T * wgt;
+                if (bsdf_sample<float>(prims[hp], type, ns, wi, u24f(ry.x), u24f(ry.y), u24f(ry.z), wo, wgt)) T = T * wgt;
                 else alive = false;
             }
         }
