@@ -127,7 +127,10 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // 1,455 / 597, keep 4: 1,612 / 655, 8: 1,639 / 674, 12: 1,638 / 682, 16:
 // 1,616 / 684, 24: 1,543 / 662 (more keep, more shading steps with fewer
 // lanes); with the drained 512-record shadow rings: keep 8: 1,753 / 709,
-// 12: 1,738 / 717, 16: 1,704 / 715.
+// 12: 1,738 / 717, 16: 1,704 / 715; with the light guide tables and the
+// primitive type in the shading record: keep 6: 1,783 / 709, 8: 1,789 /
+// 717, 10: 1,782 / 723, 12: 1,769 / 725 (refills once 12 / 16 / 24 lanes of
+// a drain are idle: 1,782 / 1,789 / 1,788 on the sphere field).
 //
 // FP64 shading (R = double) parks its direction and roulette number in FP64
 // and queues its shadow rays with FP64 contributions (four float4s per
