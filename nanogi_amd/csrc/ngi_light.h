@@ -89,6 +89,9 @@ __device__ __forceinline__ bool light_begin(const RenderArgs& a, const Cam<R>& c
                                             uint32_t qtail, bool* pushed, V3<R>& wo, R& urr) {
     const DevScene& sc = a.sc;
     if (ls.nv >= max_nv) return false;
+    // FP32: the vertex's material read up front, as in vertex_begin.
+    using MatRef = typename std::conditional<std::is_same<R, float>::value, const DevPrim<R>, const DevPrim<R>&>::type;
+    MatRef mat = prims[ls.prim];
     const DevLight<R>& L = lights_of<R>(sc)[ls.light];
     const V3<R> Le(L.Le[0], L.Le[1], L.Le[2]);
     if (DIRECT) {
@@ -108,7 +111,7 @@ __device__ __forceinline__ bool light_begin(const RenderArgs& a, const Cam<R>& c
             else if (L.type == NGI_LIGHT_POINT) fsL = Le;
             else fsL = V3<R>(R(0));
         } else {
-            fsL = bsdf_eval_nee<R, true>(prims[ls.prim], ls.type, ls.s, ls.wi, ppE);
+            fsL = bsdf_eval_nee<R, true>(mat, ls.type, ls.s, ls.wi, ppE);
         }
         if (!is_zero(fsL)) {
             R rx = 0, ry = 0;
@@ -159,7 +162,7 @@ __device__ __forceinline__ bool light_begin(const RenderArgs& a, const Cam<R>& c
         }
     } else {
         V3<R> wgt;
-        if (!bsdf_sample<R, true>(prims[ls.prim], ls.type, ls.s, ls.wi, u0, u1, uc, wo, wgt)) return false;
+        if (!bsdf_sample<R, true>(mat, ls.type, ls.s, ls.wi, u0, u1, uc, wo, wgt)) return false;
         ls.T = ls.T * wgt;
     }
     return true;

@@ -466,11 +466,17 @@ __device__ __forceinline__ bool vertex_begin(const RenderArgs& a, const Cam<R>& 
                                              uint32_t qtail, bool* pushed, V3<R>& wo, R& urr) {
     const DevScene& sc = a.sc;
     if (ps.nv >= max_nv) return false;
+    // FP32: the vertex's material, read before the light sample so that its
+    // loads go out with the light's (the shadow-ray stores between NEE and
+    // the direction sample would otherwise hold the sample's loads back).
+    // FP64 keeps reading it where it is used (a copy spills).
+    using MatRef = typename std::conditional<std::is_same<R, float>::value, const DevPrim<R>, const DevPrim<R>&>::type;
+    MatRef mat = prims[ps.prim];
     if (DIRECT) {
         const U4 rn = rng4(ps.c0, ps.c1, (uint32_t)ps.nv, NGI_STREAM_NEE, a.seed, a.stream_id);
         TraceRay sr;
         int target;
-        const V3<R> C = nee_contribution<R>(sc, cam, prims, ps.type, ps.prim, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
+        const V3<R> C = nee_contribution<R>(sc, cam, &mat, ps.type, 0, ps.s, ps.wi, ps.T, rn, a.w, a.h, ps.pixel, sr, target);
         if (QUEUE && !is_zero(C)) {
             const unsigned long long m = __ballot(1);
             shq_push(*q, qwave, qtail + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull)), sr, C, target);
@@ -491,7 +497,7 @@ __device__ __forceinline__ bool vertex_begin(const RenderArgs& a, const Cam<R>& 
         wo = camera_sample<R>(a, cam, u0, u1, ps.pixel);
     } else {
         V3<R> wgt;
-        if (!bsdf_sample<R>(prims[ps.prim], ps.type, ps.s, ps.wi, u0, u1, uc, wo, wgt)) return false;
+        if (!bsdf_sample<R>(mat, ps.type, ps.s, ps.wi, u0, u1, uc, wo, wgt)) return false;
         ps.T = ps.T * wgt;
     }
     return true;
