@@ -130,7 +130,12 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // 12: 1,738 / 717, 16: 1,704 / 715; with the light guide tables and the
 // primitive type in the shading record: keep 6: 1,783 / 709, 8: 1,789 /
 // 717, 10: 1,782 / 723, 12: 1,769 / 725 (refills once 12 / 16 / 24 lanes of
-// a drain are idle: 1,782 / 1,789 / 1,788 on the sphere field).
+// a drain are idle: 1,782 / 1,789 / 1,788 on the sphere field).  The best
+// keep follows the scene, not the tree depth: keep 8 against 12 (ptdirect,
+// 1024², 32 spp) on the glossy sphere field at 4 sizes, 29k-1.8M triangles
+// (wide BVH 6-9 levels), 2,207 / 2,108 / 2,059 / 1,776 against 2,139 / 2,052
+// / 2,004 / 1,755 M; on the diffuse nave 716 against 721 M and a diffuse
+// 4k-triangle room 1,421 against 1,468 M (both 6 levels).
 //
 // FP64 shading (R = double) parks its direction and roulette number in FP64
 // and queues its shadow rays with FP64 contributions (four float4s per
