@@ -77,6 +77,7 @@ struct DevScene {
     const uint32_t* guide_f = nullptr;
     const uint32_t* guide_d = nullptr;
     int num_lights = 0;
+    int any_glossy = 0;           // a primitive with a glossy or specular component
     int bvh_width = 2;
     uint32_t stack_entries = 0;   // LDS traversal-stack entries per lane (wide BVH depth + 1)
     uint32_t num_tris = 0;

@@ -133,6 +133,9 @@ static int upload_params(HostScene& s, DeviceScene& d) {
     d.view.lights_f = reinterpret_cast<const DevLight<float>*>(d.light_f_mem);
     d.view.lights_d = reinterpret_cast<const DevLight<double>*>(d.light_d_mem);
     d.view.num_lights = (int)s.light_prims.size();
+    d.view.any_glossy = 0;
+    for (const ngi_primitive& p : s.prims)
+        if (p.type & (NGI_PRIM_G | NGI_PRIM_S)) d.view.any_glossy = 1;
     d.params_version = s.params_version;
     return NGI_OK;
 }

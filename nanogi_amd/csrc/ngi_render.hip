@@ -135,7 +135,10 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // 1024², 32 spp) on the glossy sphere field at 4 sizes, 29k-1.8M triangles
 // (wide BVH 6-9 levels), 2,207 / 2,108 / 2,059 / 1,776 against 2,139 / 2,052
 // / 2,004 / 1,755 M; on the diffuse nave 716 against 721 M and a diffuse
-// 4k-triangle room 1,421 against 1,468 M (both 6 levels).
+// 4k-triangle room 1,421 against 1,468 M (both 6 levels).  A glossy or
+// specular vertex costs about twice a diffuse one to shade, so scenes
+// without such surfaces keep NGI_PK_KEEP_DIFFUSE: nave 720 -> 728 M, room
+// 1,421 -> 1,473 M, FP64 nave 615 -> 629 M, the glossy scenes unchanged.
 //
 // FP64 shading (R = double) parks its direction and roulette number in FP64
 // and queues its shadow rays with FP64 contributions (four float4s per
@@ -149,6 +152,9 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // / 586.
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 8
+#endif
+#ifndef NGI_PK_KEEP_DIFFUSE
+#define NGI_PK_KEEP_DIFFUSE 12
 #endif
 #ifndef NGI_PK_WAVES_FP64
 #define NGI_PK_WAVES_FP64 4
@@ -225,7 +231,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, (resume_waves<R, DIRECT, CHUNKED
             // Lanes without a ray load and store back what they parked last.
             Trav8 tv = parked_trav<NGI_MK_BLOCK>(pk);
             const RayPre r = make_pre(extension_ray(ps, parked_dir_r<NGI_MK_BLOCK, R>(pk)));
-            const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : NGI_PK_KEEP);
+            const int keep = __builtin_amdgcn_readfirstlane(feed.done() ? 0 : (a.sc.any_glossy ? NGI_PK_KEEP : NGI_PK_KEEP_DIFFUSE));
             trav8_run<COUNT>(a.sc, r, NGI_EPS_F, tv, busy, keep, tc, ext_stk, NGI_MK_BLOCK);
             park_trav<NGI_MK_BLOCK>(pk, tv);
         }
