@@ -215,9 +215,9 @@ __global__ void __launch_bounds__(NGI_ADJ_BLOCK, NGI_ADJ_WAVES) adjoint_kernel(R
         // ---- one vertex (src/nanogi.cpp:483-606 / 645-801) ----
         bool alive = nv < max_nv;
         // Vertex reflectance enters here (NEE fsE, then the BSDF weight).  (A
-        // register copy of the material read before the record stores, as the
-        // forward kernels make, spills 6 VGPRs here: 35.08 against 34.38 ms per
-        // 64-spp replay on the nave.)
+        // register copy of the material taken before the record stores, as the
+        // forward kernels take one, spills 6 VGPRs here: 35.08 against 34.38 ms
+        // per 64-spp replay on the nave.)
         if (alive && (type & (NGI_PRIM_D | NGI_PRIM_G)) && q.nrec < g.max_rec) adj_record<ZEROS>(g, gid, q, prim);
         if (alive && DIRECT && sc.num_lights > 0) {
             const U4 rn = rng4(c0, c1, (uint32_t)nv, NGI_STREAM_NEE, a.seed, a.stream_id);
