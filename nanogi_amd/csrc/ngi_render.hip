@@ -23,7 +23,9 @@ namespace ngi {
 // VGPRs, 933 at 4 waves, 948 at 6; with the shadow queue 1,447 M at 5, 1,362
 // at 4, 1,426 at 6).  ptdirect's variant for small scenes (chunked path
 // feed) gets 4, without spills: on the 36-triangle Cornell box 1,200 M
-// samples/s against 1,075 M with 5 (pt loses 1% with 4 and keeps 5).
+// samples/s against 1,075 M with 5 (pt loses 1% with 4 and keeps 5).  With
+// resumable traversals 4 waves (no spills) still lose on every scene: glossy
+// sphere fields 0.4-1.5%, the nave 1.9%, a diffuse room 0.1%.
 #ifndef NGI_PK_WAVES
 #define NGI_PK_WAVES 5
 #endif
