@@ -623,7 +623,9 @@ __device__ __forceinline__ BVert bd_vertex(const BdptCtx& c, const BVert& pv, f3
 // QUEUE (wide BVH, scenes of at least NGI_CHUNKED_FEED_TRIS triangles): the
 // subpaths' traversals are resumable and the connection rays go through
 // drained rings.  Measured (M samples/s, Cornell box / nave at 16 spp):
-// traced in place 198 / 64.1, connection rays queued 191 / 79.8.
+// traced in place 198 / 64.1, connection rays queued 191 / 79.8.  The keep
+// of the subpath traversals matters little (nave at 8 spp: 4 / 8 / 12 / 16
+// lanes: 82.1 / 82.1-82.4 / 82.6 / 82.4 M).
 #ifndef NGI_BD_KEEP
 #define NGI_BD_KEEP 8
 #endif
