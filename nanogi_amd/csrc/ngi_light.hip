@@ -95,6 +95,11 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
 #ifndef NGI_LT_WAVES_FP64
 #define NGI_LT_WAVES_FP64 4
 #endif
+// Light paths are refilled once NGI_LT_PATH_REFILL lanes wait (exact feed),
+// as the path tracer's lanes are.
+#ifndef NGI_LT_PATH_REFILL
+#define NGI_LT_PATH_REFILL 16
+#endif
 template <typename R, bool DIRECT, bool CHUNKED>
 constexpr unsigned light_resume_waves() {
     return std::is_same<R, double>::value ? NGI_LT_WAVES_FP64 : (CHUNKED && DIRECT ? NGI_LT_WAVES_SMALL : NGI_LT_WAVES);
@@ -126,7 +131,7 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, (light_resume_waves<R, DIRECT, C
             active = light_end<R>(a, prims, ls, parked_dir_r<NGI_LT_BLOCK, R>(pk), parked_urr_r<NGI_LT_BLOCK, R>(pk), parked_hit<NGI_LT_BLOCK>(pk));
         if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
-            if (need) {
+            if (need && ((uint32_t)__popcll(need) >= (CHUNKED ? 1u : (uint32_t)NGI_LT_PATH_REFILL) || need == ~0ull)) {
                 const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
                 if (!active && id != ~0ull) {
                     cnt[CNT_SAMPLES]++;

@@ -155,6 +155,20 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 #ifndef NGI_PK_KEEP
 #define NGI_PK_KEEP 8
 #endif
+// Lanes whose paths have ended take new ones (an atomic on the path counter,
+// then the camera vertex) only once NGI_PK_PATH_REFILL of them wait, so most
+// shading steps skip the refill; the chunked feed (small scenes) takes ids
+// from a wave's own chunk and refills at once.  Measured (M samples/s,
+// ptdirect, 1024², 32 spp; sphere field 29k / 90k / 336k / 1.8M triangles,
+// nave, a 4k-triangle room on the chunked feed): refill at once 2,209 /
+// 2,104 / 2,057 / 1,778 / 722 / 1,470; once 4 wait 2,306 / 2,188 / 2,134 /
+// 1,804 / 724 / 1,466; 8: 2,384 / 2,252 / 2,193 / 1,846 / 734 / 1,465; 12:
+// 2,416 / 2,270 / 2,211 / 1,865 / 743 / 1,462; 16: 2,424 / 2,280 / 2,221 /
+// 1,866 / 743 / 1,457; 24: 2,437 / 2,283 / 2,227 / 1,865 / 726 / 1,438;
+// 32: 2,434 / 2,281 / 2,231 / 1,861 / 712 / 1,402.
+#ifndef NGI_PK_PATH_REFILL
+#define NGI_PK_PATH_REFILL 16
+#endif
 #ifndef NGI_PK_KEEP_DIFFUSE
 #define NGI_PK_KEEP_DIFFUSE 12
 #endif
@@ -193,7 +207,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, (resume_waves<R, DIRECT, CHUNKED
                                                   parked_hit<NGI_MK_BLOCK>(pk));
         if (!feed.done()) {
             const unsigned long long need = __ballot(!active);
-            if (need) {
+            if (need && ((uint32_t)__popcll(need) >= (CHUNKED ? 1u : (uint32_t)NGI_PK_PATH_REFILL) || need == ~0ull)) {
                 const unsigned long long id = feed.take<CHUNKED>(next, a.path_count, need, a.path_chunk);
                 if (!active && id != ~0ull) {
                     path_start<R>(a, a.path_begin + id, cam, ps);
