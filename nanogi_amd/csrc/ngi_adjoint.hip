@@ -46,6 +46,9 @@
 namespace ngi {
 
 #define NGI_ADJ_BLOCK 256
+// Ended paths are replaced at once: waiting for 16 of them, as the forward
+// persistent kernels do, made the 64-spp replay on the nave 35.83 ms against
+// 34.58 ms.
 // Minimum waves per SIMD for the adjoint kernel's register allocation: 4
 // (128 VGPRs) measured fastest on the nave (38.2 ms per 64-spp replay against
 // 44.8 ms unconstrained at 140 VGPRs / 3 waves, 38.6 ms at 5 waves).
