@@ -140,7 +140,10 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 // 4k-triangle room 1,421 against 1,468 M (both 6 levels).  A glossy or
 // specular vertex costs about twice a diffuse one to shade, so scenes
 // without such surfaces keep NGI_PK_KEEP_DIFFUSE: nave 720 -> 728 M, room
-// 1,421 -> 1,473 M, FP64 nave 615 -> 629 M, the glossy scenes unchanged.
+// 1,421 -> 1,473 M, FP64 nave 615 -> 629 M, the glossy scenes unchanged (12
+// lanes).  With paths refilled 16 at a time (nave / room): diffuse keep 10:
+// 739 / 1,447, 12: 743 / 1,470, 16: 748 / 1,496, 20: 749 / 1,505, 24: 746 /
+// 1,504; glossy keep 6 / 8 / 10 on the sphere field: 1,861 / 1,863 / 1,867.
 //
 // FP64 shading (R = double) parks its direction and roulette number in FP64
 // and queues its shadow rays with FP64 contributions (four float4s per
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(NGI_MK_BLOCK, CHUNKED && DIRECT ? NGI_PK_WAVES
 #define NGI_PK_PATH_REFILL 16
 #endif
 #ifndef NGI_PK_KEEP_DIFFUSE
-#define NGI_PK_KEEP_DIFFUSE 12
+#define NGI_PK_KEEP_DIFFUSE 16
 #endif
 #ifndef NGI_PK_WAVES_FP64
 #define NGI_PK_WAVES_FP64 4
