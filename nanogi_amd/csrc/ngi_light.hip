@@ -96,7 +96,9 @@ __global__ void __launch_bounds__(NGI_LT_BLOCK, CHUNKED && DIRECT ? NGI_LT_WAVES
 #define NGI_LT_WAVES_FP64 4
 #endif
 // Light paths are refilled once NGI_LT_PATH_REFILL lanes wait (exact feed),
-// as the path tracer's lanes are.
+// as the path tracer's lanes are.  Measured (ltdirect, M samples/s, nave
+// FP32 / FP64, sphere field FP32): at once 737 / 612 / 2,095; 8 waiting
+// 766 / 620 / 2,098; 16: 770 / 627 / 2,095; 24: 747 / 609 / 2,105.
 #ifndef NGI_LT_PATH_REFILL
 #define NGI_LT_PATH_REFILL 16
 #endif
